@@ -1,0 +1,191 @@
+#!/usr/bin/env python3
+"""Benchmark: cell-steps/s of the InfOli network update on one MI355X.
+
+Default workload: sparse_100k (100,000 cells, degree ~ Poisson(100),
+w ~ U[0.02, 0.06], g_CaL ~ U[0.5, 1.1], 6 uA/cm^2 pulse to 20% of cells on
+steps [2000, 4000)), FP32 arithmetic with compensated gap-junction sums,
+V_axon trace every 100 steps (the reference's print granularity).  A step is
+one timestep of the whole network; --steps K of them are timed after
+--warmup W untimed ones, inputs resident in HBM.
+
+Printed (one JSON line): the metric, the roofline of the step kernel
+(algorithmic bytes per launch / mean launch time from HIP events on the
+library's stream) and the CPU baseline (the oracle's OpenMP FP32
+restatement of the reference loops, timed on this host's cores on a bounded
+sample of the same workload).
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "cell-steps/s at 100k cells, FP32+Kahan batched vs FP64 plain; % roofline"
+
+
+def algorithmic_bytes_per_step(n, nnz, prec):
+    """Bytes one step must move: per neighbour entry the sender id (int32)
+    and conductance (4 or 8 B); per cell 16 state values read and written,
+    g_CaL and Iapp read, V_dend read once, V_dend / trace written."""
+    t = 8 if prec == "fp64" else 4
+    per_entry = 4 + t
+    per_cell = 36 * t
+    return per_entry * nnz + per_cell * n
+
+
+def cpu_baseline(wl, target_s=12.0):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    n = wl["n"]
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = max(1, min(threads, int(os.environ.get("OMP_NUM_THREADS", threads))))
+
+    def run(k):
+        st = O.initial_state(n)
+        st[O.G_CAL] = wl["gcal"]
+        _, el = O.run(O.FP32, st, k, wl["row_ptr"], wl["col"], wl["w"],
+                      iapp_base=wl["iapp_base"], mask=wl["mask"], amp=wl["amp"],
+                      t0=wl["t0"], t1=wl["t1"], stride=100, nthreads=threads, trace=True)
+        return el
+
+    probe = run(2)
+    k = int(max(2, min(2000, target_s / max(probe / 2, 1e-6))))
+    el = run(k)
+    cpu = platform.processor() or platform.machine()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": n * k / el, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{wl['name']}: steps [0, {k}) of {wl['steps']}, all {n} cells, FP32 "
+                      f"OpenMP restatement of src/openmp/infoli.c:496-679 (oracle/infoli_oracle.c), "
+                      f"{threads} threads on {cpu}; {el:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=500)
+    ap.add_argument("--workload", default="sparse_100k")
+    ap.add_argument("--precision", default="fp32k", choices=["fp32k", "fp32", "fp64"])
+    ap.add_argument("--stride", type=int, default=100)
+    ap.add_argument("--mode", default="auto", choices=["auto", "graph", "persistent", "step", "tune"])
+    ap.add_argument("--lanes", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(multi-GPU sharding is described in DESIGN.md, not implemented)", file=sys.stderr)
+        sys.exit(2)
+
+    import infoli_amd as I
+    from infoli_amd import workloads as W
+
+    print(f"# building workload {args.workload}", file=sys.stderr, flush=True)
+    wl = W.build(args.workload)
+    n, nnz = wl["n"], wl.nnz
+
+    def make(prec):
+        net = I.Network(n, precision=prec)
+        W.load_into(net, wl)
+        if args.lanes:
+            net.set_exec(lanes_per_cell=args.lanes)
+        if args.mode == "graph":
+            net.set_exec(mode=I.EXEC_GRAPH)
+        elif args.mode == "persistent":
+            net.set_exec(mode=I.EXEC_PERSISTENT, steps_per_launch=500)
+        elif args.mode == "step":
+            net.set_exec(mode=I.EXEC_STEP)
+        elif args.mode == "tune":
+            net.autotune(trial_steps=200)
+        return net
+
+    def timed(net, steps, warmup):
+        if warmup:
+            net.advance(warmup, trace_stride=args.stride)
+        net.synchronize()
+        t0 = time.perf_counter()
+        net.advance(steps, trace_stride=args.stride)
+        net.synchronize()
+        return time.perf_counter() - t0
+
+    print(f"# {args.precision}: {n} cells, {nnz} neighbour entries", file=sys.stderr, flush=True)
+    net = make(args.precision)
+    exec_cfg = net.exec
+    el = timed(net, args.steps, args.warmup)
+    value = n * args.steps / el
+    dev_ms = net.last_timing()["advance_ms"]
+
+    # kernel timing: HIP events around every step-kernel launch
+    net.set_profiling(True)
+    prof_steps = min(args.steps, 2000)
+    net.advance(prof_steps)
+    tm = net.last_timing()
+    net.set_profiling(False)
+    launches = max(tm["kernel_launches"], 1)
+    mean_launch_ms = tm["kernel_ms"] / launches
+    steps_per_launch = prof_steps / launches
+    bytes_per_launch = algorithmic_bytes_per_step(n, nnz, args.precision) * steps_per_launch
+    achieved = bytes_per_launch / (mean_launch_ms * 1e-3) / 1e9
+    net.close()
+
+    extra = {"device_ms": round(dev_ms, 3), "exec": exec_cfg,
+             "mean_kernel_launch_ms": round(mean_launch_ms, 5),
+             "steps_per_kernel_launch": steps_per_launch}
+    if not args.no_fp64 and args.precision != "fp64":
+        print("# fp64 comparison run", file=sys.stderr, flush=True)
+        n64 = make("fp64")
+        k64 = min(args.steps, 2000)
+        el64 = timed(n64, k64, min(args.warmup, 200))
+        n64.close()
+        extra["fp64_cell_steps_per_s"] = n * k64 / el64
+        extra["speedup_vs_fp64"] = value / extra["fp64_cell_steps_per_s"]
+
+    cpu = None
+    if not args.no_cpu:
+        print("# CPU baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(wl)
+        extra["gpu_over_cpu"] = value / cpu["value"]
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32",
+        "data": "synthetic (seeded counter-based generators)",
+        "config": {"workload": wl["name"], "cells": n, "neighbour_entries": nnz,
+                   "mean_degree": round(nnz / n, 3), "precision": {
+                       "fp32k": "fp32 + compensated gap-junction sums", "fp32": "fp32",
+                       "fp64": "fp64"}[args.precision],
+                   "trace_stride": args.stride, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": bytes_per_launch,
+                     "kernel": "k_step (fused gap-junction + compartment step)"},
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

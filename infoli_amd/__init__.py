@@ -1,0 +1,35 @@
+"""infoli_amd -- MI355X-native InfOli network update.
+
+Python mirror of the C ABI in include/infoli_gpu.h (ctypes over
+infoli_amd/lib/libinfoli_gpu.so).  The compute path is the HIP library; this
+module only moves arrays across the boundary.  If the library is missing the
+import of a compute entry point raises -- there is no CPU fallback.
+"""
+from .network import (  # noqa: F401
+    FP32,
+    FP32_KAHAN,
+    FP64,
+    EXEC_AUTO,
+    EXEC_BATCHED,
+    EXEC_GRAPH,
+    EXEC_PERSISTENT,
+    EXEC_STEP,
+    VARS,
+    InfoliError,
+    Network,
+    Params,
+    gen_directed,
+    gen_mask,
+    gen_normal,
+    gen_symmetric,
+    gen_uniform,
+    library,
+    library_path,
+    read_conn_dense,
+    read_conn_triplets,
+    read_iapp_file,
+    read_values,
+)
+from . import workloads  # noqa: F401
+
+__version__ = "0.1.0"

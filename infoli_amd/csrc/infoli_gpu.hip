@@ -1,0 +1,1021 @@
+/*
+ * infoli_gpu.hip -- C ABI runtime of the MI355X InfOli network step.
+ *
+ * Owns the device copies of one network (see infoli_kernels.h for the
+ * layout), launches the step kernels in one of three ways (one kernel per
+ * step, per-step kernels replayed as a hipGraph, or a persistent multi-step
+ * kernel with a grid barrier) and moves traces back to the host.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "infoli_gpu.h"
+#include "infoli_host.h"
+#include "infoli_kernels.h"
+
+using namespace infoli;
+
+#define HIPCHK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess)                                                              \
+            return infoli_fail(INFOLI_ERR_HIP, "%s failed: %s (%s:%d)", #x,                \
+                               hipGetErrorString(e_), __FILE__, __LINE__);                 \
+    } while (0)
+
+#define RCHK(x)                          \
+    do {                                 \
+        int r_ = (x);                    \
+        if (r_ != INFOLI_OK) return r_;  \
+    } while (0)
+
+namespace {
+
+constexpr size_t kTraceRingBytes = size_t(1) << 30;   // device trace ring budget
+constexpr size_t kVldsMaxBytes = 96 * 1024;           // V_dend staged in LDS up to this size
+
+int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+struct GraphKey {
+    int mode = -1, L = 0, vlds = 0, G = 0, grid = 0;
+    size_t args_len = 0;
+    unsigned char args[sizeof(StepArgs<double>)];
+};
+
+}  // namespace
+
+struct infoli_ctx {
+    int device = 0;
+    int prec = INFOLI_FP64;
+    int64_t n = 0, npad = 0;
+    size_t tsize = 8;
+    infoli_params params;
+    hipStream_t stream = nullptr;
+
+    void *d_state = nullptr;
+    void *d_vd[2] = {nullptr, nullptr};
+    int32_t *d_deg = nullptr;
+    int64_t *d_slice_ptr = nullptr;
+    int32_t *d_col = nullptr;
+    void *d_w = nullptr;
+    int sell_L = 0;                 // lanes per cell of the layout on the device
+    int64_t sell_entries = 0;
+
+    void *d_iapp_base = nullptr;
+    uint8_t *d_mask = nullptr;
+    double amp = 0;
+    int64_t t0 = 0, t1 = 0;
+    void *d_tab = nullptr;
+    int64_t tab_t0 = 0, tab_n = 0;
+
+    void *d_trace = nullptr;
+    int64_t trace_cap = 0;
+    void *h_stage = nullptr;        // pinned staging for trace / state copies
+    size_t h_stage_bytes = 0;
+
+    int64_t *d_step = nullptr;
+    GridBarrier *d_bar = nullptr;
+    int *d_flag = nullptr;
+    int64_t step = 0;
+
+    std::vector<int64_t> row_ptr;
+    std::vector<int32_t> col;
+    std::vector<double> w;
+    int max_deg = 0;
+    std::vector<int64_t> part;
+
+    infoli_exec exec{};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    GraphKey gkey;
+
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> prof_ev;
+    bool profiling = false;
+    double last_ms = 0, kern_ms = 0;
+    int64_t kern_launches = 0;
+
+    int num_cus = 0;
+};
+
+/* ------------------------------------------------------------------------ */
+/* kernel dispatch                                                           */
+/* ------------------------------------------------------------------------ */
+
+namespace {
+
+template <typename T>
+Model<T> make_model(const infoli_params &p);
+
+template <>
+Model<float> make_model<float>(const infoli_params &p)
+{
+    const float *f = p.f;
+    Model<float> m;
+    m.DELTA = f[0]; m.C_M = f[2]; m.G_NA_S = f[3]; m.G_KDR_S = f[4]; m.G_K_S = f[5];
+    m.G_LS = f[6]; m.G_K_CA = f[7]; m.G_CAH = f[8]; m.G_LD = f[9]; m.G_H = f[10];
+    m.G_NA_A = f[11]; m.G_K_A = f[13]; m.G_LA = f[14];
+    m.V_NA = f[18]; m.V_K = f[19]; m.V_CA = f[20]; m.V_H = f[21]; m.V_L = f[22];
+    volatile float G_INT = f[17], P1 = f[15], P2 = f[16];   // float arithmetic, as OMP:587-666
+    m.G_sd = G_INT / (1 - P1);
+    m.G_ds = G_INT / P1;
+    m.G_as = G_INT / (1 - P2);
+    m.G_sa = G_INT / P2;
+    return m;
+}
+
+template <>
+Model<double> make_model<double>(const infoli_params &p)
+{
+    const double *v = p.v;
+    Model<double> m;
+    m.DELTA = v[0]; m.C_M = v[2]; m.G_NA_S = v[3]; m.G_KDR_S = v[4]; m.G_K_S = v[5];
+    m.G_LS = v[6]; m.G_K_CA = v[7]; m.G_CAH = v[8]; m.G_LD = v[9]; m.G_H = v[10];
+    m.G_NA_A = v[11]; m.G_K_A = v[13]; m.G_LA = v[14];
+    m.V_NA = v[18]; m.V_K = v[19]; m.V_CA = v[20]; m.V_H = v[21]; m.V_L = v[22];
+    m.G_sd = v[17] / (1 - v[15]);
+    m.G_ds = v[17] / v[15];
+    m.G_as = v[17] / (1 - v[16]);
+    m.G_sa = v[17] / v[16];
+    return m;
+}
+
+template <typename T>
+StepArgs<T> make_args(const infoli_ctx *c, int64_t stride)
+{
+    StepArgs<T> a;
+    memset(&a, 0, sizeof a);
+    a.n = c->n;
+    a.npad = c->npad;
+    a.st = (T *)c->d_state;
+    a.vd[0] = (T *)c->d_vd[0];
+    a.vd[1] = (T *)c->d_vd[1];
+    a.deg = c->d_deg;
+    a.slice_ptr = c->d_slice_ptr;
+    a.col = c->d_col;
+    a.w = (const T *)c->d_w;
+    a.iapp_base = (const T *)c->d_iapp_base;
+    a.mask = c->d_mask;
+    a.amp = (T)c->amp;
+    a.t0 = c->t0;
+    a.t1 = c->t1;
+    a.tab = (const T *)c->d_tab;
+    a.tab_t0 = c->tab_t0;
+    a.tab_n = c->tab_n;
+    a.trace = stride > 0 ? (T *)c->d_trace : nullptr;
+    a.stride = stride > 0 ? stride : 1;
+    a.trace_cap = c->trace_cap > 0 ? c->trace_cap : 1;
+    a.d_step = nullptr;
+    a.m = make_model<T>(c->params);
+    return a;
+}
+
+typedef void (*step_fn_f)(StepArgs<float>, int64_t);
+typedef void (*step_fn_d)(StepArgs<double>, int64_t);
+typedef void (*pers_fn_f)(StepArgs<float>, GridBarrier *, int);
+typedef void (*pers_fn_d)(StepArgs<double>, GridBarrier *, int);
+typedef void (*gap_fn_f)(StepArgs<float>, int64_t, float *);
+typedef void (*gap_fn_d)(StepArgs<double>, int64_t, double *);
+
+int lidx(int L)
+{
+    switch (L) {
+    case 1: return 0;
+    case 2: return 1;
+    case 4: return 2;
+    case 8: return 3;
+    case 16: return 4;
+    default: return -1;
+    }
+}
+
+#define INFOLI_L_TABLE(KERN, T, ...) \
+    {(const void *)KERN<T, 1, __VA_ARGS__>, (const void *)KERN<T, 2, __VA_ARGS__>, \
+     (const void *)KERN<T, 4, __VA_ARGS__>, (const void *)KERN<T, 8, __VA_ARGS__>, \
+     (const void *)KERN<T, 16, __VA_ARGS__>}
+
+/* [precision][vlds][L] */
+const void *const k_step_tab[3][2][5] = {
+    {INFOLI_L_TABLE(k_step, double, false, false), INFOLI_L_TABLE(k_step, double, false, true)},
+    {INFOLI_L_TABLE(k_step, float, false, false), INFOLI_L_TABLE(k_step, float, false, true)},
+    {INFOLI_L_TABLE(k_step, float, true, false), INFOLI_L_TABLE(k_step, float, true, true)},
+};
+const void *const k_persist_tab[3][2][5] = {
+    {INFOLI_L_TABLE(k_persist, double, false, false), INFOLI_L_TABLE(k_persist, double, false, true)},
+    {INFOLI_L_TABLE(k_persist, float, false, false), INFOLI_L_TABLE(k_persist, float, false, true)},
+    {INFOLI_L_TABLE(k_persist, float, true, false), INFOLI_L_TABLE(k_persist, float, true, true)},
+};
+const void *const k_gap_tab[3][5] = {
+    INFOLI_L_TABLE(k_gap_currents, double, false),
+    INFOLI_L_TABLE(k_gap_currents, float, false),
+    INFOLI_L_TABLE(k_gap_currents, float, true),
+};
+
+bool is_f32(const infoli_ctx *c) { return c->prec != INFOLI_FP64; }
+
+size_t vlds_bytes(const infoli_ctx *c) { return c->exec.v_in_lds ? (size_t)c->n * c->tsize : 0; }
+
+int launch_step(infoli_ctx *c, const void *args, int64_t k)
+{
+    const void *fn = k_step_tab[c->prec][c->exec.v_in_lds ? 1 : 0][lidx(c->exec.lanes_per_cell)];
+    const int64_t nblk = c->npad / kBlock;
+    int64_t kk = k;
+    void *kargs[] = {const_cast<void *>(args), &kk};
+    HIPCHK(hipLaunchKernel(fn, dim3((unsigned)nblk), dim3(kBlock), kargs, vlds_bytes(c), c->stream));
+    return INFOLI_OK;
+}
+
+int persist_grid(infoli_ctx *c)
+{
+    const void *fn = k_persist_tab[c->prec][c->exec.v_in_lds ? 1 : 0][lidx(c->exec.lanes_per_cell)];
+    int per_cu = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock, vlds_bytes(c)));
+    if (per_cu < 1) return infoli_fail(INFOLI_ERR_HIP, "persistent kernel does not fit on a CU");
+    const int64_t nblk = c->npad / kBlock;
+    int64_t g = (int64_t)per_cu * c->num_cus;
+    return (int)std::min<int64_t>(g, nblk);
+}
+
+int launch_persist(infoli_ctx *c, const void *args, int nsteps, int grid)
+{
+    const void *fn = k_persist_tab[c->prec][c->exec.v_in_lds ? 1 : 0][lidx(c->exec.lanes_per_cell)];
+    GridBarrier *gb = c->d_bar;
+    int ns = nsteps;
+    void *kargs[] = {const_cast<void *>(args), &gb, &ns};
+    hipLaunchKernelGGL(k_reset_barrier, dim3(1), dim3(1), 0, c->stream, c->d_bar);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipLaunchKernel(fn, dim3((unsigned)grid), dim3(kBlock), kargs, vlds_bytes(c), c->stream));
+    return INFOLI_OK;
+}
+
+__global__ void k_set_step(int64_t *d_step, int64_t v) { *d_step = v; }
+
+/* ------------------------------------------------------------------------ */
+/* sliced-ELL layout                                                         */
+/* ------------------------------------------------------------------------ */
+
+template <typename T>
+int upload_sell(infoli_ctx *c, int L)
+{
+    const int CPW = 64 / L;
+    const int64_t n = c->n, npad = c->npad;
+    const int64_t nslices = npad / CPW;
+    std::vector<int32_t> deg(npad, 0);
+    for (int64_t i = 0; i < n; i++) deg[i] = (int32_t)(c->row_ptr[i + 1] - c->row_ptr[i]);
+    std::vector<int64_t> sp(nslices + 1, 0);
+    for (int64_t s = 0; s < nslices; s++) {
+        int md = 0;
+        for (int j = 0; j < CPW; j++) md = std::max(md, deg[s * CPW + j]);
+        sp[s + 1] = sp[s] + (int64_t)((md + L - 1) / L) * 64;
+    }
+    const int64_t ne = sp[nslices];
+    std::vector<int32_t> col(std::max<int64_t>(ne, 1), 0);
+    std::vector<T> w(std::max<int64_t>(ne, 1), T(0));
+    for (int64_t i = 0; i < n; i++) {
+        const int64_t s = i / CPW, cw = i % CPW, r0 = c->row_ptr[i];
+        for (int p = 0; p < deg[i]; p++) {
+            const int64_t e = sp[s] + (int64_t)(p / L) * 64 + cw * L + (p % L);
+            col[e] = c->col[r0 + p];
+            w[e] = (T)c->w[r0 + p];
+        }
+    }
+    (void)hipFree(c->d_col); c->d_col = nullptr;
+    (void)hipFree(c->d_w); c->d_w = nullptr;
+    (void)hipFree(c->d_slice_ptr); c->d_slice_ptr = nullptr;
+    HIPCHK(hipMalloc(&c->d_col, sizeof(int32_t) * col.size()));
+    HIPCHK(hipMalloc(&c->d_w, sizeof(T) * w.size()));
+    HIPCHK(hipMalloc(&c->d_slice_ptr, sizeof(int64_t) * sp.size()));
+    HIPCHK(hipMemcpy(c->d_col, col.data(), sizeof(int32_t) * col.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_w, w.data(), sizeof(T) * w.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_slice_ptr, sp.data(), sizeof(int64_t) * sp.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_deg, deg.data(), sizeof(int32_t) * npad, hipMemcpyHostToDevice));
+    c->sell_L = L;
+    c->sell_entries = ne;
+    return INFOLI_OK;
+}
+
+int ensure_layout(infoli_ctx *c)
+{
+    if (c->sell_L == c->exec.lanes_per_cell) return INFOLI_OK;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return is_f32(c) ? upload_sell<float>(c, c->exec.lanes_per_cell)
+                     : upload_sell<double>(c, c->exec.lanes_per_cell);
+}
+
+void drop_graph(infoli_ctx *c)
+{
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+    c->gkey.mode = -1;
+}
+
+int choose_lanes(double mean_deg)
+{
+    if (mean_deg < 12) return 1;
+    if (mean_deg < 40) return 2;
+    if (mean_deg < 160) return 4;
+    if (mean_deg < 640) return 8;
+    return 16;
+}
+
+void default_exec(infoli_ctx *c)
+{
+    const double mean_deg = c->n > 0 ? (double)c->col.size() / (double)c->n : 0;
+    c->exec.mode = INFOLI_EXEC_GRAPH;
+    c->exec.lanes_per_cell = choose_lanes(mean_deg);
+    c->exec.block = kBlock;
+    c->exec.steps_per_launch = 100;
+    c->exec.v_in_lds = ((size_t)c->n * c->tsize <= kVldsMaxBytes && mean_deg >= 64) ? 1 : 0;
+}
+
+int ensure_stage(infoli_ctx *c, size_t bytes)
+{
+    if (c->h_stage_bytes >= bytes) return INFOLI_OK;
+    if (c->h_stage) (void)hipHostFree(c->h_stage);
+    c->h_stage = nullptr;
+    c->h_stage_bytes = 0;
+    HIPCHK(hipHostMalloc(&c->h_stage, bytes, hipHostMallocDefault));
+    c->h_stage_bytes = bytes;
+    return INFOLI_OK;
+}
+
+int validate_ctx(const infoli_t *c, const char *who)
+{
+    if (!c) return infoli_fail(INFOLI_ERR_ARG, "%s: null context", who);
+    return INFOLI_OK;
+}
+
+/* device T[n] <- double[n] */
+int upload_values(infoli_ctx *c, void *dst, const double *src, int64_t n)
+{
+    RCHK(ensure_stage(c, (size_t)n * c->tsize));
+    if (is_f32(c)) {
+        float *s = (float *)c->h_stage;
+        for (int64_t i = 0; i < n; i++) s[i] = (float)src[i];
+    } else {
+        memcpy(c->h_stage, src, sizeof(double) * n);
+    }
+    HIPCHK(hipMemcpyAsync(dst, c->h_stage, (size_t)n * c->tsize, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return INFOLI_OK;
+}
+
+int download_values(infoli_ctx *c, const void *src, double *dst, int64_t n)
+{
+    RCHK(ensure_stage(c, (size_t)n * c->tsize));
+    HIPCHK(hipMemcpyAsync(c->h_stage, src, (size_t)n * c->tsize, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (is_f32(c)) {
+        const float *s = (const float *)c->h_stage;
+        for (int64_t i = 0; i < n; i++) dst[i] = (double)s[i];
+    } else {
+        memcpy(dst, c->h_stage, sizeof(double) * n);
+    }
+    return INFOLI_OK;
+}
+
+}  // namespace
+
+/* ------------------------------------------------------------------------ */
+/* lifecycle                                                                 */
+/* ------------------------------------------------------------------------ */
+
+extern "C" const char *infoli_version(void) { return "infoli-mi355x 0.1 (gfx950)"; }
+
+extern "C" int infoli_create(infoli_t **out, const infoli_params *params, int64_t n_cells,
+                             int precision, int device)
+{
+    infoli_clear_error();
+    if (!out) return infoli_fail(INFOLI_ERR_ARG, "infoli_create: null output");
+    *out = nullptr;
+    if (!params) return infoli_fail(INFOLI_ERR_ARG, "infoli_create: null params");
+    if (n_cells <= 0 || n_cells > INT32_MAX)
+        return infoli_fail(INFOLI_ERR_ARG, "infoli_create: n_cells = %lld out of range",
+                           (long long)n_cells);
+    if (precision < INFOLI_FP64 || precision > INFOLI_FP32_KAHAN)
+        return infoli_fail(INFOLI_ERR_ARG, "infoli_create: unknown precision %d", precision);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return infoli_fail(INFOLI_ERR_NODEVICE, "infoli_create: no HIP device visible");
+    if (device < 0 || device >= ndev)
+        return infoli_fail(INFOLI_ERR_NODEVICE, "infoli_create: device %d of %d", device, ndev);
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return infoli_fail(INFOLI_ERR_NODEVICE, "infoli_create: device %d is %s, built for gfx950",
+                           device, prop.gcnArchName);
+    HIPCHK(hipSetDevice(device));
+
+    infoli_ctx *c = new (std::nothrow) infoli_ctx();
+    if (!c) return infoli_fail(INFOLI_ERR_NOMEM, "infoli_create: no host memory");
+    c->device = device;
+    c->prec = precision;
+    c->n = n_cells;
+    c->npad = round_up(n_cells, kBlock);
+    c->tsize = precision == INFOLI_FP64 ? 8 : 4;
+    c->params = *params;
+    c->num_cus = prop.multiProcessorCount;
+    c->row_ptr.assign(n_cells + 1, 0);
+
+    int rc = INFOLI_OK;
+    auto fail = [&](int code) {
+        infoli_destroy(c);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+        return fail(infoli_fail(INFOLI_ERR_HIP, "infoli_create: stream/event creation failed"));
+    const size_t plane = (size_t)c->npad * c->tsize;
+    if (hipMalloc(&c->d_state, plane * kNV) != hipSuccess ||
+        hipMalloc(&c->d_vd[0], plane) != hipSuccess || hipMalloc(&c->d_vd[1], plane) != hipSuccess ||
+        hipMalloc(&c->d_deg, sizeof(int32_t) * c->npad) != hipSuccess ||
+        hipMalloc(&c->d_step, sizeof(int64_t)) != hipSuccess ||
+        hipMalloc(&c->d_bar, sizeof(GridBarrier)) != hipSuccess ||
+        hipMalloc(&c->d_flag, sizeof(int)) != hipSuccess)
+        return fail(infoli_fail(INFOLI_ERR_NOMEM, "infoli_create: device allocation of %lld cells failed",
+                                (long long)n_cells));
+    if (hipMemset(c->d_state, 0, plane * kNV) != hipSuccess ||
+        hipMemset(c->d_vd[0], 0, plane) != hipSuccess || hipMemset(c->d_vd[1], 0, plane) != hipSuccess)
+        return fail(infoli_fail(INFOLI_ERR_HIP, "infoli_create: memset failed"));
+
+    // initial state (OMP:267-292): float readings for FP32, double for FP64
+    std::vector<double> vals(n_cells);
+    for (int v = 0; v < kNV; v++) {
+        const double x = precision == INFOLI_FP64 ? params->v[INFOLI_P_STATE0 + v]
+                                                  : (double)params->f[INFOLI_P_STATE0 + v];
+        std::fill(vals.begin(), vals.end(), x);
+        void *dst = (v == V_DEND) ? c->d_vd[0] : (char *)c->d_state + (size_t)v * plane;
+        rc = upload_values(c, dst, vals.data(), n_cells);
+        if (rc) return fail(rc);
+    }
+    default_exec(c);
+    rc = ensure_layout(c);
+    if (rc) return fail(rc);
+    *out = c;
+    return INFOLI_OK;
+}
+
+extern "C" void infoli_destroy(infoli_t *c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graph(c);
+    for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    void *bufs[] = {c->d_state, c->d_vd[0], c->d_vd[1], c->d_deg, c->d_slice_ptr, c->d_col, c->d_w,
+                    c->d_iapp_base, c->d_mask, c->d_tab, c->d_trace, c->d_step, c->d_bar, c->d_flag};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    if (c->h_stage) (void)hipHostFree(c->h_stage);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+/* ------------------------------------------------------------------------ */
+/* inputs                                                                    */
+/* ------------------------------------------------------------------------ */
+
+extern "C" int infoli_set_connectivity_csr(infoli_t *c, const int64_t *row_ptr,
+                                           const int32_t *col, const double *w)
+{
+    infoli_clear_error();
+    RCHK(validate_ctx(c, "infoli_set_connectivity_csr"));
+    if (!row_ptr) return infoli_fail(INFOLI_ERR_ARG, "infoli_set_connectivity_csr: null row_ptr");
+    const int64_t n = c->n;
+    if (row_ptr[0] != 0) return infoli_fail(INFOLI_ERR_ARG, "row_ptr[0] = %lld, must be 0", (long long)row_ptr[0]);
+    int max_deg = 0;
+    for (int64_t i = 0; i < n; i++) {
+        const int64_t d = row_ptr[i + 1] - row_ptr[i];
+        if (d < 0 || d > INT32_MAX)
+            return infoli_fail(INFOLI_ERR_ARG, "row_ptr not non-decreasing at cell %lld", (long long)i);
+        max_deg = std::max<int>(max_deg, (int)d);
+    }
+    const int64_t nnz = row_ptr[n];
+    if (nnz > 0 && (!col || !w))
+        return infoli_fail(INFOLI_ERR_ARG, "infoli_set_connectivity_csr: null col or w with %lld entries",
+                           (long long)nnz);
+    for (int64_t e = 0; e < nnz; e++) {
+        if (col[e] < 0 || col[e] >= n)
+            return infoli_fail(INFOLI_ERR_ARG, "col[%lld] = %d outside [0, %lld)", (long long)e, col[e],
+                               (long long)n);
+        if (!std::isfinite(w[e]))
+            return infoli_fail(INFOLI_ERR_ARG, "w[%lld] is not finite", (long long)e);
+    }
+    for (size_t k = 0; k + 1 < c->part.size(); k++) {
+        for (int64_t i = c->part[k]; i < c->part[k + 1]; i++)
+            for (int64_t e = row_ptr[i]; e < row_ptr[i + 1]; e++)
+                if (col[e] < c->part[k] || col[e] >= c->part[k + 1])
+                    return infoli_fail(INFOLI_ERR_ARG,
+                                       "junction %d -> %lld crosses the partition boundary of network %zu",
+                                       col[e], (long long)i, k);
+    }
+    c->row_ptr.assign(row_ptr, row_ptr + n + 1);
+    c->col.assign(col ? col : nullptr, col ? col + nnz : nullptr);
+    c->w.assign(w ? w : nullptr, w ? w + nnz : nullptr);
+    c->max_deg = max_deg;
+    const int keep_mode = c->exec.mode;
+    default_exec(c);
+    if (keep_mode != INFOLI_EXEC_GRAPH) c->exec.mode = keep_mode;
+    c->sell_L = 0;
+    drop_graph(c);
+    return ensure_layout(c);
+}
+
+extern "C" int infoli_set_partition(infoli_t *c, int n_nets, const int64_t *offsets)
+{
+    infoli_clear_error();
+    RCHK(validate_ctx(c, "infoli_set_partition"));
+    if (n_nets <= 0 || !offsets) return infoli_fail(INFOLI_ERR_ARG, "infoli_set_partition: bad arguments");
+    if (offsets[0] != 0 || offsets[n_nets] != c->n)
+        return infoli_fail(INFOLI_ERR_ARG, "partition must start at 0 and end at n_cells");
+    for (int k = 0; k < n_nets; k++)
+        if (offsets[k + 1] <= offsets[k])
+            return infoli_fail(INFOLI_ERR_ARG, "network %d is empty or offsets decrease", k);
+    for (int k = 0; k < n_nets; k++)
+        for (int64_t i = offsets[k]; i < offsets[k + 1]; i++)
+            for (int64_t e = c->row_ptr[i]; e < c->row_ptr[i + 1]; e++)
+                if (c->col[e] < offsets[k] || c->col[e] >= offsets[k + 1])
+                    return infoli_fail(INFOLI_ERR_ARG,
+                                       "junction %d -> %lld crosses the boundary of network %d",
+                                       c->col[e], (long long)i, k);
+    c->part.assign(offsets, offsets + n_nets + 1);
+    return INFOLI_OK;
+}
+
+extern "C" int infoli_set_gcal(infoli_t *c, const double *g)
+{
+    infoli_clear_error();
+    RCHK(validate_ctx(c, "infoli_set_gcal"));
+    return infoli_set_state(c, INFOLI_G_CAL, g);
+}
+
+extern "C" int infoli_set_iapp(infoli_t *c, const double *base)
+{
+    infoli_clear_error();
+    RCHK(validate_ctx(c, "infoli_set_iapp"));
+    HIPCHK(hipSetDevice(c->device));
+    if (!base) {
+        if (c->d_iapp_base) HIPCHK(hipFree(c->d_iapp_base));
+        c->d_iapp_base = nullptr;
+        drop_graph(c);
+        return INFOLI_OK;
+    }
+    for (int64_t i = 0; i < c->n; i++)
+        if (!std::isfinite(base[i])) return infoli_fail(INFOLI_ERR_ARG, "iapp base[%lld] not finite", (long long)i);
+    if (!c->d_iapp_base) HIPCHK(hipMalloc(&c->d_iapp_base, (size_t)c->n * c->tsize));
+    drop_graph(c);
+    return upload_values(c, c->d_iapp_base, base, c->n);
+}
+
+extern "C" int infoli_set_iapp_pulse(infoli_t *c, const uint8_t *mask, double amp, int64_t t0, int64_t t1)
+{
+    infoli_clear_error();
+    RCHK(validate_ctx(c, "infoli_set_iapp_pulse"));
+    if (!std::isfinite(amp)) return infoli_fail(INFOLI_ERR_ARG, "pulse amplitude not finite");
+    HIPCHK(hipSetDevice(c->device));
+    if (mask) {
+        if (!c->d_mask) HIPCHK(hipMalloc(&c->d_mask, (size_t)c->n));
+        HIPCHK(hipMemcpy(c->d_mask, mask, (size_t)c->n, hipMemcpyHostToDevice));
+    } else if (c->d_mask) {
+        HIPCHK(hipFree(c->d_mask));
+        c->d_mask = nullptr;
+    }
+    c->amp = amp;
+    c->t0 = t0;
+    c->t1 = t1;
+    drop_graph(c);
+    return INFOLI_OK;
+}
+
+extern "C" int infoli_set_iapp_table(infoli_t *c, int64_t t_first, int64_t n_steps, const double *table)
+{
+    infoli_clear_error();
+    RCHK(validate_ctx(c, "infoli_set_iapp_table"));
+    HIPCHK(hipSetDevice(c->device));
+    if (c->d_tab) HIPCHK(hipFree(c->d_tab));
+    c->d_tab = nullptr;
+    c->tab_n = 0;
+    drop_graph(c);
+    if (n_steps <= 0) return INFOLI_OK;
+    if (!table || t_first < 0) return infoli_fail(INFOLI_ERR_ARG, "infoli_set_iapp_table: bad table");
+    const int64_t cnt = n_steps * c->n;
+    HIPCHK(hipMalloc(&c->d_tab, (size_t)cnt * c->tsize));
+    RCHK(upload_values(c, c->d_tab, table, cnt));
+    c->tab_t0 = t_first;
+    c->tab_n = n_steps;
+    return INFOLI_OK;
+}
+
+static void *var_ptr(infoli_ctx *c, int var)
+{
+    if (var == V_DEND) return c->d_vd[c->step & 1];
+    return (char *)c->d_state + (size_t)var * c->npad * c->tsize;
+}
+
+extern "C" int infoli_set_state(infoli_t *c, int var, const double *values)
+{
+    infoli_clear_error();
+    RCHK(validate_ctx(c, "infoli_set_state"));
+    if (var < 0 || var >= INFOLI_NVARS || !values)
+        return infoli_fail(INFOLI_ERR_ARG, "infoli_set_state: bad variable %d or null values", var);
+    HIPCHK(hipSetDevice(c->device));
+    return upload_values(c, var_ptr(c, var), values, c->n);
+}
+
+extern "C" int infoli_get_state(infoli_t *c, int var, double *values)
+{
+    infoli_clear_error();
+    RCHK(validate_ctx(c, "infoli_get_state"));
+    if (var < 0 || var >= INFOLI_NVARS || !values)
+        return infoli_fail(INFOLI_ERR_ARG, "infoli_get_state: bad variable %d or null values", var);
+    HIPCHK(hipSetDevice(c->device));
+    return download_values(c, var_ptr(c, var), values, c->n);
+}
+
+/* ------------------------------------------------------------------------ */
+/* execution                                                                 */
+/* ------------------------------------------------------------------------ */
+
+extern "C" int infoli_get_exec(const infoli_t *c, infoli_exec *out)
+{
+    RCHK(validate_ctx(c, "infoli_get_exec"));
+    if (!out) return infoli_fail(INFOLI_ERR_ARG, "infoli_get_exec: null output");
+    *out = c->exec;
+    return INFOLI_OK;
+}
+
+extern "C" int infoli_set_exec(infoli_t *c, const infoli_exec *e)
+{
+    infoli_clear_error();
+    RCHK(validate_ctx(c, "infoli_set_exec"));
+    if (!e) return infoli_fail(INFOLI_ERR_ARG, "infoli_set_exec: null exec");
+    infoli_exec x = *e;
+    if (x.mode == INFOLI_EXEC_AUTO) {
+        default_exec(c);
+        x = c->exec;
+    }
+    if (x.mode < INFOLI_EXEC_STEP || x.mode > INFOLI_EXEC_PERSISTENT)
+        return infoli_fail(INFOLI_ERR_ARG, "infoli_set_exec: mode %d not available", x.mode);
+    if (lidx(x.lanes_per_cell) < 0)
+        return infoli_fail(INFOLI_ERR_ARG, "lanes_per_cell %d not in {1,2,4,8,16}", x.lanes_per_cell);
+    if (x.block != kBlock) return infoli_fail(INFOLI_ERR_ARG, "block must be %d", kBlock);
+    if (x.steps_per_launch < 1 || x.steps_per_launch > 100000)
+        return infoli_fail(INFOLI_ERR_ARG, "steps_per_launch %d out of range", x.steps_per_launch);
+    if (x.v_in_lds && (size_t)c->n * c->tsize > kVldsMaxBytes)
+        return infoli_fail(INFOLI_ERR_ARG, "v_in_lds: %lld cells do not fit the LDS budget", (long long)c->n);
+    c->exec = x;
+    drop_graph(c);
+    return ensure_layout(c);
+}
+
+extern "C" int infoli_set_profiling(infoli_t *c, int on)
+{
+    RCHK(validate_ctx(c, "infoli_set_profiling"));
+    c->profiling = on != 0;
+    return INFOLI_OK;
+}
+
+extern "C" int64_t infoli_trace_records(const infoli_t *c, int64_t n_steps, int64_t stride)
+{
+    if (!c || n_steps <= 0 || stride <= 0) return 0;
+    const int64_t a = c->step, b = c->step + n_steps;   // records: multiples of stride in [a, b)
+    return (b - 1) / stride - (a + stride - 1) / stride + 1;
+}
+
+extern "C" int64_t infoli_step_count(const infoli_t *c) { return c ? c->step : -1; }
+
+namespace {
+
+int prof_begin(infoli_ctx *c)
+{
+    if (!c->profiling) return INFOLI_OK;
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    c->prof_ev.push_back(a);
+    c->prof_ev.push_back(b);
+    HIPCHK(hipEventRecord(a, c->stream));
+    return INFOLI_OK;
+}
+
+int prof_end(infoli_ctx *c)
+{
+    if (!c->profiling) return INFOLI_OK;
+    HIPCHK(hipEventRecord(c->prof_ev.back(), c->stream));
+    return INFOLI_OK;
+}
+
+template <typename T>
+int capture_graph(infoli_ctx *c, const StepArgs<T> &a, int mode, int G, int grid)
+{
+    GraphKey key;
+    key.mode = mode;
+    key.L = c->exec.lanes_per_cell;
+    key.vlds = c->exec.v_in_lds;
+    key.G = G;
+    key.grid = grid;
+    key.args_len = sizeof a;
+    memcpy(key.args, &a, sizeof a);
+    if (c->gexec && c->gkey.mode == key.mode && c->gkey.L == key.L && c->gkey.vlds == key.vlds &&
+        c->gkey.G == key.G && c->gkey.grid == key.grid && c->gkey.args_len == key.args_len &&
+        memcmp(c->gkey.args, key.args, key.args_len) == 0)
+        return INFOLI_OK;
+    drop_graph(c);
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = INFOLI_OK;
+    if (mode == INFOLI_EXEC_GRAPH) {
+        for (int k = 0; k < G && rc == INFOLI_OK; k++) rc = launch_step(c, &a, k);
+    } else {
+        rc = launch_persist(c, &a, G, grid);
+    }
+    if (rc == INFOLI_OK) {
+        hipLaunchKernelGGL(k_bump_step, dim3(1), dim3(1), 0, c->stream, c->d_step, (int64_t)G);
+        if (hipGetLastError() != hipSuccess) rc = infoli_fail(INFOLI_ERR_HIP, "bump launch failed in capture");
+    }
+    hipGraph_t g = nullptr;
+    hipError_t ee = hipStreamEndCapture(c->stream, &g);
+    if (rc != INFOLI_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (ee != hipSuccess) return infoli_fail(INFOLI_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ee));
+    c->graph = g;
+    HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+    c->gkey = key;
+    return INFOLI_OK;
+}
+
+/* steps [t, t+count) in the context's execution mode */
+template <typename T>
+int run_steps(infoli_ctx *c, StepArgs<T> a, int64_t t, int64_t count)
+{
+    int mode = c->exec.mode;
+    if (c->profiling && mode == INFOLI_EXEC_GRAPH) mode = INFOLI_EXEC_STEP;
+    if (mode == INFOLI_EXEC_STEP) {
+        a.d_step = nullptr;
+        for (int64_t k = 0; k < count; k++) {
+            RCHK(prof_begin(c));
+            RCHK(launch_step(c, &a, t + k));
+            RCHK(prof_end(c));
+            if (c->profiling) c->kern_launches++;
+        }
+        return INFOLI_OK;
+    }
+    const int G = c->exec.steps_per_launch;
+    a.d_step = c->d_step;
+    hipLaunchKernelGGL(k_set_step, dim3(1), dim3(1), 0, c->stream, c->d_step, t);
+    HIPCHK(hipGetLastError());
+    int64_t done = 0;
+    if (mode == INFOLI_EXEC_GRAPH) {
+        if (count >= G) RCHK(capture_graph(c, a, mode, G, 0));
+        for (; done + G <= count; done += G) HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+        StepArgs<T> b = a;
+        b.d_step = nullptr;
+        for (; done < count; done++) RCHK(launch_step(c, &b, t + done));
+        return INFOLI_OK;
+    }
+    // persistent
+    int grid = persist_grid(c);
+    if (grid <= 0) return grid < 0 ? grid : infoli_fail(INFOLI_ERR_HIP, "no persistent grid");
+    if (!c->profiling && count >= G) RCHK(capture_graph(c, a, mode, G, grid));
+    while (done < count) {
+        const int piece = (int)std::min<int64_t>(G, count - done);
+        if (!c->profiling && piece == G && c->gexec) {
+            HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+        } else {
+            RCHK(prof_begin(c));
+            RCHK(launch_persist(c, &a, piece, grid));
+            RCHK(prof_end(c));
+            if (c->profiling) c->kern_launches++;
+            hipLaunchKernelGGL(k_bump_step, dim3(1), dim3(1), 0, c->stream, c->d_step, (int64_t)piece);
+            HIPCHK(hipGetLastError());
+        }
+        done += piece;
+    }
+    return INFOLI_OK;
+}
+
+template <typename T>
+int copy_records(infoli_ctx *c, int64_t t, int64_t count, int64_t stride, double *trace, int64_t *rec)
+{
+    const int64_t first = (t + stride - 1) / stride, last = (t + count - 1) / stride;
+    if (last < first) return INFOLI_OK;
+    const int64_t nrec = last - first + 1, n = c->n;
+    RCHK(ensure_stage(c, (size_t)nrec * n * sizeof(T)));
+    for (int64_t r = 0; r < nrec;) {
+        const int64_t slot = (first + r) % c->trace_cap;
+        const int64_t run = std::min(nrec - r, c->trace_cap - slot);
+        HIPCHK(hipMemcpyAsync((T *)c->h_stage + r * n, (T *)c->d_trace + slot * n, sizeof(T) * run * n,
+                              hipMemcpyDeviceToHost, c->stream));
+        r += run;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const T *s = (const T *)c->h_stage;
+    double *dst = trace + (*rec) * n;
+    for (int64_t i = 0; i < nrec * n; i++) dst[i] = (double)s[i];
+    *rec += nrec;
+    return INFOLI_OK;
+}
+
+template <typename T>
+int advance_impl(infoli_ctx *c, int64_t n_steps, int64_t stride, double *trace)
+{
+    const int64_t n = c->n;
+    const int64_t stride_eff = trace ? stride : 0;
+    int64_t cap = 1;
+    if (stride_eff > 0) {
+        const int64_t want = infoli_trace_records(c, n_steps, stride_eff);
+        const int64_t budget = std::max<int64_t>(1, (int64_t)(kTraceRingBytes / ((size_t)n * sizeof(T))));
+        cap = std::max<int64_t>(1, std::min(want, budget));
+        if (cap > c->trace_cap) {
+            HIPCHK(hipStreamSynchronize(c->stream));
+            if (c->d_trace) HIPCHK(hipFree(c->d_trace));
+            c->d_trace = nullptr;
+            c->trace_cap = 0;
+            HIPCHK(hipMalloc(&c->d_trace, (size_t)cap * n * sizeof(T)));
+            c->trace_cap = cap;
+            drop_graph(c);
+        }
+    }
+    StepArgs<T> a = make_args<T>(c, stride_eff);
+    c->kern_ms = 0;
+    c->kern_launches = 0;
+    for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
+    c->prof_ev.clear();
+
+    hipLaunchKernelGGL(k_reset_barrier, dim3(1), dim3(1), 0, c->stream, c->d_bar);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    int64_t t = c->step, left = n_steps, rec = 0;
+    while (left > 0) {
+        int64_t chunk = left;
+        if (stride_eff > 0) chunk = std::min<int64_t>(chunk, c->trace_cap * stride_eff);
+        RCHK(run_steps<T>(c, a, t, chunk));
+        if (stride_eff > 0) RCHK(copy_records<T>(c, t, chunk, stride_eff, trace, &rec));
+        t += chunk;
+        left -= chunk;
+        c->step = t;
+    }
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, c->stream, c->d_flag);
+    const int64_t plane = c->npad;
+    hipLaunchKernelGGL(k_check_finite<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                       (const T *)c->d_vd[c->step & 1], (const T *)c->d_state + V_SOMA * plane,
+                       (const T *)c->d_state + V_AXON * plane, n, c->d_flag);
+    HIPCHK(hipGetLastError());
+    int flag = 0;
+    GridBarrier gb{0, 0};
+    HIPCHK(hipMemcpyAsync(&flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&gb, c->d_bar, sizeof gb, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    if (c->profiling) {
+        for (size_t i = 0; i + 1 < c->prof_ev.size(); i += 2) {
+            float e = 0;
+            HIPCHK(hipEventElapsedTime(&e, c->prof_ev[i], c->prof_ev[i + 1]));
+            c->kern_ms += e;
+        }
+    }
+    if (gb.error) return infoli_fail(INFOLI_ERR_BARRIER, "persistent kernel grid barrier timed out");
+    if (flag) return infoli_fail(INFOLI_ERR_NONFINITE, "a voltage is NaN or Inf after step %lld", (long long)c->step);
+    return INFOLI_OK;
+}
+
+}  // namespace
+
+extern "C" int infoli_advance(infoli_t *c, int64_t n_steps, int64_t trace_stride, double *trace)
+{
+    infoli_clear_error();
+    RCHK(validate_ctx(c, "infoli_advance"));
+    if (n_steps < 0) return infoli_fail(INFOLI_ERR_ARG, "infoli_advance: n_steps < 0");
+    if (trace && trace_stride <= 0) return infoli_fail(INFOLI_ERR_ARG, "infoli_advance: trace_stride must be > 0");
+    if (n_steps == 0) return INFOLI_OK;
+    HIPCHK(hipSetDevice(c->device));
+    RCHK(ensure_layout(c));
+    return is_f32(c) ? advance_impl<float>(c, n_steps, trace_stride, trace)
+                     : advance_impl<double>(c, n_steps, trace_stride, trace);
+}
+
+extern "C" int infoli_gap_currents(infoli_t *c, double *i_c)
+{
+    infoli_clear_error();
+    RCHK(validate_ctx(c, "infoli_gap_currents"));
+    if (!i_c) return infoli_fail(INFOLI_ERR_ARG, "infoli_gap_currents: null output");
+    HIPCHK(hipSetDevice(c->device));
+    RCHK(ensure_layout(c));
+    const int64_t nblk = c->npad / kBlock;
+    void *out = nullptr;
+    HIPCHK(hipMalloc(&out, (size_t)c->n * c->tsize));
+    const void *fn = k_gap_tab[c->prec][lidx(c->exec.lanes_per_cell)];
+    int64_t t = c->step;
+    int rc = INFOLI_OK;
+    if (is_f32(c)) {
+        StepArgs<float> a = make_args<float>(c, 0);
+        void *kargs[] = {&a, &t, &out};
+        if (hipLaunchKernel(fn, dim3((unsigned)nblk), dim3(kBlock), kargs, 0, c->stream) != hipSuccess)
+            rc = infoli_fail(INFOLI_ERR_HIP, "gap-current kernel launch failed");
+    } else {
+        StepArgs<double> a = make_args<double>(c, 0);
+        void *kargs[] = {&a, &t, &out};
+        if (hipLaunchKernel(fn, dim3((unsigned)nblk), dim3(kBlock), kargs, 0, c->stream) != hipSuccess)
+            rc = infoli_fail(INFOLI_ERR_HIP, "gap-current kernel launch failed");
+    }
+    if (rc == INFOLI_OK) rc = download_values(c, out, i_c, c->n);
+    (void)hipFree(out);
+    return rc;
+}
+
+extern "C" int infoli_last_timing(const infoli_t *c, double *advance_ms, double *kernel_ms,
+                                  int64_t *kernel_launches)
+{
+    RCHK(validate_ctx(c, "infoli_last_timing"));
+    if (advance_ms) *advance_ms = c->last_ms;
+    if (kernel_ms) *kernel_ms = c->kern_ms;
+    if (kernel_launches) *kernel_launches = c->kern_launches;
+    return INFOLI_OK;
+}
+
+extern "C" int infoli_synchronize(infoli_t *c)
+{
+    RCHK(validate_ctx(c, "infoli_synchronize"));
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return INFOLI_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* autotuner                                                                 */
+/* ------------------------------------------------------------------------ */
+
+extern "C" int infoli_autotune(infoli_t *c, int64_t trial_steps, infoli_exec *chosen)
+{
+    infoli_clear_error();
+    RCHK(validate_ctx(c, "infoli_autotune"));
+    if (trial_steps < 1) return infoli_fail(INFOLI_ERR_ARG, "infoli_autotune: trial_steps < 1");
+    HIPCHK(hipSetDevice(c->device));
+    // save the full state so the trials leave no trace
+    const size_t plane = (size_t)c->npad * c->tsize;
+    std::vector<unsigned char> saved(plane * (kNV + 1));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(saved.data(), c->d_state, plane * kNV, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(saved.data() + plane * kNV, c->d_vd[c->step & 1], plane, hipMemcpyDeviceToHost));
+    const int64_t step0 = c->step;
+    const infoli_exec orig = c->exec;
+    const bool prof = c->profiling;
+    c->profiling = false;
+
+    std::vector<infoli_exec> cands;
+    const int Ls[] = {1, 2, 4, 8, 16};
+    const int modes[] = {INFOLI_EXEC_GRAPH, INFOLI_EXEC_PERSISTENT};
+    for (int mode : modes)
+        for (int L : Ls)
+            for (int vl = 0; vl < 2; vl++) {
+                if (vl && (size_t)c->n * c->tsize > kVldsMaxBytes) continue;
+                infoli_exec e{mode, L, kBlock, (int)std::min<int64_t>(trial_steps, 500), vl};
+                cands.push_back(e);
+            }
+    double best = 1e300;
+    infoli_exec best_e = orig;
+    int rc = INFOLI_OK;
+    for (const infoli_exec &e : cands) {
+        c->exec = e;
+        drop_graph(c);
+        rc = ensure_layout(c);
+        if (rc) break;
+        // warm-up launch (graph capture, code load), then the timed trial
+        int r1 = infoli_advance(c, e.steps_per_launch, 0, nullptr);
+        int r2 = (r1 == INFOLI_OK || r1 == INFOLI_ERR_NONFINITE) ? infoli_advance(c, trial_steps, 0, nullptr) : r1;
+        if ((r2 == INFOLI_OK || r2 == INFOLI_ERR_NONFINITE) && c->last_ms < best) {
+            best = c->last_ms;
+            best_e = e;
+        }
+        if (r1 == INFOLI_ERR_BARRIER || r2 == INFOLI_ERR_BARRIER) {
+            rc = INFOLI_ERR_BARRIER;
+            break;
+        }
+    }
+    // restore
+    c->exec = best_e;
+    drop_graph(c);
+    c->step = step0;
+    HIPCHK(hipMemcpy(c->d_state, saved.data(), plane * kNV, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_vd[c->step & 1], saved.data() + plane * kNV, plane, hipMemcpyHostToDevice));
+    c->profiling = prof;
+    RCHK(ensure_layout(c));
+    if (rc != INFOLI_OK) return infoli_fail(rc, "infoli_autotune: a trial failed (%s)", infoli_strerror(rc));
+    if (chosen) *chosen = best_e;
+    return INFOLI_OK;
+}

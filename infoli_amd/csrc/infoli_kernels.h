@@ -1,0 +1,597 @@
+/*
+ * infoli_kernels.h -- device code of the InfOli network step for gfx950.
+ *
+ * One workgroup of 256 threads owns 256 consecutive cells.  A step is
+ *   (1) gap-junction currents: L lanes share one cell's neighbour list; a
+ *       wavefront covers 64/L cells per round and the workgroup runs L
+ *       rounds, so all 256 cells get their I_c, which goes to LDS;
+ *   (2) compartment update: one thread per cell, state in SoA planes, new
+ *       V_dend written to the other half of a ping-pong buffer.
+ * Neighbour lists are stored sliced-ELL: a slice is the 64/L cells one
+ * wavefront handles in one round; entry p of cell c in slice s lives at
+ * slice_ptr[s] + (p / L) * 64 + c * L + (p % L), so each load instruction of
+ * a wavefront reads 64 consecutive entries.
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace infoli {
+
+constexpr int kBlock = 256;          // threads per workgroup = cells per workgroup
+constexpr int kNV = 19;              // per-cell variables (default.conf 24-42 order)
+
+enum Var {
+    V_DEND = 0, CALCIUM_R, POTASSIUM_S, HCURRENT_Q, CA2PLUS, I_CAH, G_CAL,
+    V_SOMA, SODIUM_M, SODIUM_H, POTASSIUM_N, POTASSIUM_P, POTASSIUM_X_S,
+    CALCIUM_K, CALCIUM_L, V_AXON, SODIUM_M_A, SODIUM_H_A, POTASSIUM_X_A
+};
+
+/* Model constants in the step's arithmetic.  The four coupling
+ * conductances are the reference's G_INT/(1-P1), G_INT/P1, G_INT/(1-P2)
+ * and G_INT/P2, evaluated once on the host in the same precision. */
+template <typename T>
+struct Model {
+    T DELTA, C_M, G_NA_S, G_KDR_S, G_K_S, G_LS, G_K_CA, G_CAH, G_LD, G_H,
+      G_NA_A, G_K_A, G_LA, V_NA, V_K, V_CA, V_H, V_L;
+    T G_sd, G_ds, G_as, G_sa;
+};
+
+template <typename T>
+struct StepArgs {
+    int64_t n;            // cells
+    int64_t npad;         // plane stride (multiple of kBlock)
+    T *st;                // kNV planes of npad values
+    T *vd[2];             // V_dend ping-pong: vd[t & 1] holds V_dend at the start of step t
+    const int32_t *deg;   // neighbour count per cell
+    const int64_t *slice_ptr;
+    const int32_t *col;   // sender ids, sliced-ELL
+    const T *w;           // conductances, sliced-ELL
+    const T *iapp_base;   // per-cell constant Iapp (nullable)
+    const uint8_t *mask;  // pulse mask (nullable = every cell)
+    T amp;
+    int64_t t0, t1;       // pulse on steps [t0, t1)
+    const T *tab;         // per-step Iapp rows (nullable)
+    int64_t tab_t0, tab_n;
+    T *trace;             // ring of trace_cap records of n values (nullable)
+    int64_t stride, trace_cap;
+    const int64_t *d_step;// device step counter (nullable: step index is absolute)
+    Model<T> m;
+};
+
+struct GridBarrier {
+    unsigned int count;
+    unsigned int error;
+};
+
+/* ---------------------------------------------------------------------- */
+/* compartment update, FP32: src/openmp/infoli.c:556-676                   */
+/* ---------------------------------------------------------------------- */
+
+struct CellState32 {
+    float vd, r, s, q, ca, icah, gcal, vs, m, h, n, p, xs, k, l, va, ma, ha, xa;
+};
+struct CellState64 {
+    double vd, r, s, q, ca, icah, gcal, vs, m, h, n, p, xs, k, l, va, ma, ha, xa;
+};
+
+__device__ __forceinline__ float fmin_ref(float a, float b) { return (a < b) ? a : b; }
+__device__ __forceinline__ double fmin_ref(double a, double b) { return (a < b) ? a : b; }
+
+__device__ __forceinline__ void ode(const Model<float> &P, CellState32 &c, float I_c, float iApp)
+{
+    const float Vd = c.vd, Vs = c.vs, Va = c.va;
+    // Dend H current OMP:561-564
+    float q_inf = 1 / (1 + expf((Vd + 80) / 4));
+    float tau_q = 1 / (expf(-0.086f * Vd - 14.6f) + expf(0.070f * Vd - 1.87f));
+    float dq_dt = (q_inf - c.q) / tau_q;
+    c.q = P.DELTA * dq_dt + c.q;
+    // Dend Ca current OMP:567-572
+    float alpha_r = 1.7f / (1 + expf(-(Vd - 5) / 13.9f));
+    float beta_r = 0.02f * (Vd + 8.5f) / (expf((Vd + 8.5f) / 5) - 1);
+    float r_inf = alpha_r / (alpha_r + beta_r);
+    float tau_r = 5 / (alpha_r + beta_r);
+    float dr_dt = (r_inf - c.r) / tau_r;
+    c.r = P.DELTA * dr_dt + c.r;
+    // Dend K current OMP:575-580 (previous Ca2Plus)
+    float alpha_s = fmin_ref((0.00002f * c.ca), 0.01f);
+    float beta_s = 0.015f;
+    float s_inf = alpha_s / (alpha_s + beta_s);
+    float tau_s = 1 / (alpha_s + beta_s);
+    float ds_dt = (s_inf - c.s) / tau_s;
+    c.s = P.DELTA * ds_dt + c.s;
+    // Dend Cal OMP:583-584 (previous I_CaH)
+    float dCa_dt = -3 * c.icah - 0.075f * c.ca;
+    c.ca = P.DELTA * dCa_dt + c.ca;
+    // dendritic currents OMP:587-593 (new gates, old voltages)
+    float I_sd = P.G_sd * (Vd - Vs);
+    float I_CaH = P.G_CAH * (c.r * c.r) * (Vd - P.V_CA);
+    float I_K_Ca = P.G_K_CA * c.s * (Vd - P.V_K);
+    float I_ld = P.G_LD * (Vd - P.V_L);
+    float I_h = P.G_H * c.q * (Vd - P.V_H);
+    float dVd_dt = (-(I_CaH + I_sd + I_ld + I_K_Ca + I_c + I_h) + iApp) / P.C_M;
+    c.icah = I_CaH;
+    // Soma calcium OMP:599-606
+    float k_inf = (1 / (1 + expf(-1 * (Vs + 61) / 4.2f)));
+    float l_inf = (1 / (1 + expf((Vs + 85.5f) / 8.5f)));
+    float tau_l = ((20 * expf((Vs + 160) / 30) / (1 + expf((Vs + 84) / 7.3f))) + 35);
+    float dk_dt = (k_inf - c.k) / 1.0f;
+    float dl_dt = (l_inf - c.l) / tau_l;
+    c.k = P.DELTA * dk_dt + c.k;
+    c.l = P.DELTA * dl_dt + c.l;
+    // Soma sodium OMP:609-614
+    float m_inf = 1 / (1 + (expf((-30 - Vs) / 5.5f)));
+    float h_inf = 1 / (1 + (expf((-70 - Vs) / -5.8f)));
+    float tau_h = 3 * expf((-40 - Vs) / 33);
+    float dh_dt = (h_inf - c.h) / tau_h;
+    c.m = m_inf;
+    c.h = c.h + P.DELTA * dh_dt;
+    // Soma potassium OMP:617-624
+    float n_inf = 1 / (1 + expf((-3 - Vs) / 10));
+    float p_inf = 1 / (1 + expf((-51 - Vs) / -12));
+    float tau_n = 5 + (47 * expf(-(-50 - Vs) / 900));
+    float dn_dt = (n_inf - c.n) / tau_n;
+    float dp_dt = (p_inf - c.p) / tau_n;
+    c.n = P.DELTA * dn_dt + c.n;
+    c.p = P.DELTA * dp_dt + c.p;
+    // Soma potassium X OMP:627-632 (literal 0.05f)
+    float alpha_x_s = 0.13f * (Vs + 25) / (1 - expf(-(Vs + 25) / 10));
+    float beta_x_s = 1.69f * expf(-0.0125f * (Vs + 35));
+    float x_inf_s = alpha_x_s / (alpha_x_s + beta_x_s);
+    float tau_x_s = 1 / (alpha_x_s + beta_x_s);
+    float dx_dt_s = (x_inf_s - c.xs) / tau_x_s;
+    c.xs = 0.05f * dx_dt_s + c.xs;
+    // somatic currents OMP:635-642
+    float I_ds = P.G_ds * (Vs - Vd);
+    float k3 = c.k * c.k * c.k, m3 = c.m * c.m * c.m;
+    float n2 = c.n * c.n, xs2 = c.xs * c.xs;
+    float I_CaL = c.gcal * k3 * c.l * (Vs - P.V_CA);
+    float I_Na_s = P.G_NA_S * m3 * c.h * (Vs - P.V_NA);
+    float I_ls = P.G_LS * (Vs - P.V_L);
+    float I_Kdr_s = P.G_KDR_S * (n2 * n2) * (Vs - P.V_K);
+    float I_K_s = P.G_K_S * (xs2 * xs2) * (Vs - P.V_K);
+    float I_as = P.G_as * (Vs - Va);
+    float dVs_dt = (-(I_CaL + I_ds + I_as + I_Na_s + I_ls + I_Kdr_s + I_K_s)) / P.C_M;
+    // Axon sodium OMP:648-653
+    float m_inf_a = 1 / (1 + (expf((-30 - Va) / 5.5f)));
+    float h_inf_a = 1 / (1 + (expf((-60 - Va) / -5.8f)));
+    float tau_h_a = 1.5f * expf((-40 - Va) / 33);
+    float dh_dt_a = (h_inf_a - c.ha) / tau_h_a;
+    c.ma = m_inf_a;
+    c.ha = c.ha + P.DELTA * dh_dt_a;
+    // Axon potassium OMP:656-661
+    float alpha_x_a = 0.13f * (Va + 25) / (1 - expf(-(Va + 25) / 10));
+    float beta_x_a = 1.69f * expf(-0.0125f * (Va + 35));
+    float x_inf_a = alpha_x_a / (alpha_x_a + beta_x_a);
+    float tau_x_a = 1 / (alpha_x_a + beta_x_a);
+    float dx_dt_a = (x_inf_a - c.xa) / tau_x_a;
+    c.xa = 0.05f * dx_dt_a + c.xa;
+    // axonal currents OMP:664-668
+    float ma3 = c.ma * c.ma * c.ma, xa2 = c.xa * c.xa;
+    float I_Na_a = P.G_NA_A * ma3 * c.ha * (Va - P.V_NA);
+    float I_la = P.G_LA * (Va - P.V_L);
+    float I_sa = P.G_sa * (Va - Vs);
+    float I_K_a = P.G_K_A * (xa2 * xa2) * (Va - P.V_K);
+    float dVa_dt = (-(I_K_a + I_sa + I_la + I_Na_a)) / P.C_M;
+    // new voltages OMP:673-675
+    c.vd = P.DELTA * dVd_dt + Vd;
+    c.vs = P.DELTA * dVs_dt + Vs;
+    c.va = P.DELTA * dVa_dt + Va;
+}
+
+/* ---------------------------------------------------------------------- */
+/* compartment update, FP64: src/single_threaded/infoli.c:371-858          */
+/* ---------------------------------------------------------------------- */
+
+__device__ __forceinline__ void ode(const Model<double> &P, CellState64 &c, double I_c, double I_app)
+{
+    const double prevV_dend = c.vd, prevV_soma = c.vs, prevV_axon = c.va;
+    // DendHCurr ST:430-447
+    double q_inf = 1 / (1 + exp((prevV_dend + 80) / 4));
+    double tau_q = 1 / (exp(-0.086 * prevV_dend - 14.6) + exp(0.070 * prevV_dend - 1.87));
+    double dq_dt = (q_inf - c.q) / tau_q;
+    double q = P.DELTA * dq_dt + c.q;
+    // DendCaCurr ST:449-468
+    double alpha_r = 1.7 / (1 + exp(-(prevV_dend - 5) / 13.9));
+    double beta_r = 0.02 * (prevV_dend + 8.5) / (exp((prevV_dend + 8.5) / 5) - 1);
+    double r_inf = alpha_r / (alpha_r + beta_r);
+    double tau_r = 5 / (alpha_r + beta_r);
+    double dr_dt = (r_inf - c.r) / tau_r;
+    double r = P.DELTA * dr_dt + c.r;
+    // DendKCurr ST:470-489
+    double alpha_s = fmin_ref((0.00002 * c.ca), 0.01);
+    double beta_s = 0.015;
+    double s_inf = alpha_s / (alpha_s + beta_s);
+    double tau_s = 1 / (alpha_s + beta_s);
+    double ds_dt = (s_inf - c.s) / tau_s;
+    double sK = P.DELTA * ds_dt + c.s;
+    // DendCal ST:492-507
+    double dCa_dt = -3 * c.icah - 0.075 * c.ca;
+    double Ca = P.DELTA * dCa_dt + c.ca;
+    // DendCurrVolt ST:509-542
+    double I_sd = P.G_sd * (prevV_dend - prevV_soma);
+    double I_CaH = P.G_CAH * r * r * (prevV_dend - P.V_CA);
+    double I_K_Ca = P.G_K_CA * sK * (prevV_dend - P.V_K);
+    double I_ld = P.G_LD * (prevV_dend - P.V_L);
+    double I_h = P.G_H * q * (prevV_dend - P.V_H);
+    double dVd_dt = (-(I_CaH + I_sd + I_ld + I_K_Ca + I_c + I_h) + I_app) / P.C_M;
+    // SomaCalcium ST:618-640
+    double k_inf = (1 / (1 + exp(-1 * (prevV_soma + 61) / 4.2)));
+    double l_inf = (1 / (1 + exp((prevV_soma + 85.5) / 8.5)));
+    double tau_l = ((20 * exp((prevV_soma + 160) / 30) / (1 + exp((prevV_soma + 84) / 7.3))) + 35);
+    double dk_dt = (k_inf - c.k) / 1.0;
+    double dl_dt = (l_inf - c.l) / tau_l;
+    double k = P.DELTA * dk_dt + c.k;
+    double l = P.DELTA * dl_dt + c.l;
+    // SomaSodium ST:642-663
+    double m_inf = 1 / (1 + (exp((-30 - prevV_soma) / 5.5)));
+    double h_inf = 1 / (1 + (exp((-70 - prevV_soma) / -5.8)));
+    double tau_h = 3 * exp((-40 - prevV_soma) / 33);
+    double dh_dt = (h_inf - c.h) / tau_h;
+    double m = m_inf;
+    double h = c.h + P.DELTA * dh_dt;
+    // SomaPotassium ST:665-688
+    double n_inf = 1 / (1 + exp((-3 - prevV_soma) / 10));
+    double p_inf = 1 / (1 + exp((-51 - prevV_soma) / -12));
+    double tau_n = 5 + (47 * exp(-(-50 - prevV_soma) / 900));
+    double dn_dt = (n_inf - c.n) / tau_n;
+    double dp_dt = (p_inf - c.p) / tau_n;
+    double n = P.DELTA * dn_dt + c.n;
+    double p = P.DELTA * dp_dt + c.p;
+    // SomaPotassiumX ST:690-709 (literal 0.05)
+    double alpha_x_s = 0.13 * (prevV_soma + 25) / (1 - exp(-(prevV_soma + 25) / 10));
+    double beta_x_s = 1.69 * exp(-0.0125 * (prevV_soma + 35));
+    double x_inf_s = alpha_x_s / (alpha_x_s + beta_x_s);
+    double tau_x_s = 1 / (alpha_x_s + beta_x_s);
+    double dx_dt_s = (x_inf_s - c.xs) / tau_x_s;
+    double x_s = 0.05 * dx_dt_s + c.xs;
+    // SomaCurrVolt ST:711-749
+    double I_ds = P.G_ds * (prevV_soma - prevV_dend);
+    double I_CaL = c.gcal * k * k * k * l * (prevV_soma - P.V_CA);
+    double I_Na_s = P.G_NA_S * m * m * m * h * (prevV_soma - P.V_NA);
+    double I_ls = P.G_LS * (prevV_soma - P.V_L);
+    double I_Kdr_s = P.G_KDR_S * n * n * n * n * (prevV_soma - P.V_K);
+    double xs2 = x_s * x_s;
+    double I_K_s = P.G_K_S * (xs2 * xs2) * (prevV_soma - P.V_K);
+    double I_as = P.G_as * (prevV_soma - prevV_axon);
+    double dVs_dt = (-(I_CaL + I_ds + I_as + I_Na_s + I_ls + I_Kdr_s + I_K_s)) / P.C_M;
+    // AxonSodium ST:788-810
+    double m_inf_a = 1 / (1 + (exp((-30 - prevV_axon) / 5.5)));
+    double h_inf_a = 1 / (1 + (exp((-60 - prevV_axon) / -5.8)));
+    double tau_h_a = 1.5 * exp((-40 - prevV_axon) / 33);
+    double dh_dt_a = (h_inf_a - c.ha) / tau_h_a;
+    double m_a = m_inf_a;
+    double h_a = c.ha + P.DELTA * dh_dt_a;
+    // AxonPotassium ST:812-831
+    double alpha_x_a = 0.13 * (prevV_axon + 25) / (1 - exp(-(prevV_axon + 25) / 10));
+    double beta_x_a = 1.69 * exp(-0.0125 * (prevV_axon + 35));
+    double x_inf_a = alpha_x_a / (alpha_x_a + beta_x_a);
+    double tau_x_a = 1 / (alpha_x_a + beta_x_a);
+    double dx_dt_a = (x_inf_a - c.xa) / tau_x_a;
+    double x_a = 0.05 * dx_dt_a + c.xa;
+    // AxonCurrVolt ST:833-858
+    double I_Na_a = P.G_NA_A * m_a * m_a * m_a * h_a * (prevV_axon - P.V_NA);
+    double I_la = P.G_LA * (prevV_axon - P.V_L);
+    double I_sa = P.G_sa * (prevV_axon - prevV_soma);
+    double xa2 = x_a * x_a;
+    double I_K_a = P.G_K_A * (xa2 * xa2) * (prevV_axon - P.V_K);
+    double dVa_dt = (-(I_K_a + I_sa + I_la + I_Na_a)) / P.C_M;
+
+    c.q = q; c.r = r; c.s = sK; c.ca = Ca; c.icah = I_CaH;
+    c.k = k; c.l = l; c.m = m; c.h = h; c.n = n; c.p = p; c.xs = x_s;
+    c.ma = m_a; c.ha = h_a; c.xa = x_a;
+    c.vd = P.DELTA * dVd_dt + prevV_dend;
+    c.vs = P.DELTA * dVs_dt + prevV_soma;
+    c.va = P.DELTA * dVa_dt + prevV_axon;
+}
+
+template <typename T> struct CellOf;
+template <> struct CellOf<float> { using type = CellState32; };
+template <> struct CellOf<double> { using type = CellState64; };
+
+/* ---------------------------------------------------------------------- */
+/* gap-junction term and accumulators                                       */
+/* ---------------------------------------------------------------------- */
+
+/* w * f(V) * V with f(V) = 0.8 exp(-V^2/100) + 0.2 (OMP:514-517, ST:549-552).
+ * FP32: exp(-V^2/100) = exp2(V^2 * (-log2(e)/100)) on the hardware exp2
+ * unit (one multiply, one v_exp_f32); FP64: libm-accurate exp. */
+__device__ __forceinline__ float gj_term(float vi, float vj, float w)
+{
+    const float K = -0.014426950408889634f; /* -log2(e) / 100 */
+    float V = vi - vj;
+    float e = __builtin_amdgcn_exp2f((V * V) * K);
+    float f = 0.8f * e + 0.2f;
+    return w * f * V;
+}
+
+__device__ __forceinline__ double gj_term(double vi, double vj, double w)
+{
+    double V = vi - vj;
+    double f = 0.8 * exp(-1 * (V * V) / 100) + 0.2;
+    return w * f * V;
+}
+
+template <typename T, bool KAHAN>
+struct Acc {
+    T s = 0, c = 0;
+    __device__ __forceinline__ void add(T x)
+    {
+        if constexpr (KAHAN) {
+            T t = s + x;                       // TwoSum (Knuth): exact error of s + x
+            T bp = t - s;
+            c += (s - (t - bp)) + (x - bp);
+            s = t;
+        } else {
+            s += x;
+        }
+    }
+    __device__ __forceinline__ void merge(T os, T oc)
+    {
+        if constexpr (KAHAN) {
+            T t = s + os;
+            T bp = t - s;
+            T e = (s - (t - bp)) + (os - bp);
+            c = c + oc + e;
+            s = t;
+        } else {
+            s += os;
+        }
+    }
+    __device__ __forceinline__ T value() const
+    {
+        if constexpr (KAHAN) return s + c;
+        else return s;
+    }
+};
+
+__device__ __forceinline__ float shfl_xor(float v, int m) { return __shfl_xor(v, m, 64); }
+__device__ __forceinline__ double shfl_xor(double v, int m) { return __shfl_xor(v, m, 64); }
+
+/* ---------------------------------------------------------------------- */
+/* one workgroup, one step                                                  */
+/* ---------------------------------------------------------------------- */
+
+template <typename T>
+__device__ __forceinline__ T iapp_of(const StepArgs<T> &a, int64_t cell, int64_t t)
+{
+    if (a.tab && t >= a.tab_t0 && t < a.tab_t0 + a.tab_n)
+        return a.tab[(t - a.tab_t0) * a.n + cell];
+    T b = a.iapp_base ? a.iapp_base[cell] : T(0);
+    bool on = (t >= a.t0) && (t < a.t1) && (!a.mask || a.mask[cell]);
+    return on ? b + a.amp : b;
+}
+
+template <typename T, int L, bool KAHAN, bool VLDS>
+__device__ __forceinline__ void step_block(const StepArgs<T> &a, int64_t blk, int64_t t,
+                                           T *ic_lds, T *v_lds)
+{
+    constexpr int CPW = 64 / L;             // cells per wavefront per round
+    constexpr int CPR = kBlock / L;         // cells per workgroup per round
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int cw = lane / L, l = lane % L;
+    const T *__restrict__ vin = a.vd[t & 1];
+    T *__restrict__ vout = a.vd[(t & 1) ^ 1];
+    const int64_t cell0 = blk * kBlock;
+
+    if constexpr (VLDS) {
+        for (int64_t i = tid; i < a.n; i += kBlock) v_lds[i] = vin[i];
+        __syncthreads();
+    }
+    const T *__restrict__ vsrc = VLDS ? v_lds : vin;
+
+    // (1) gap-junction currents, L rounds of CPR cells
+#pragma unroll
+    for (int r = 0; r < L; r++) {
+        const int local = r * CPR + wave * CPW + cw;
+        const int64_t cell = cell0 + local;
+        const int64_t slice = (cell0 + r * CPR + wave * CPW) / CPW;
+        Acc<T, KAHAN> acc;
+        if (cell < a.n) {
+            const int d = a.deg[cell];
+            if (d > 0) {
+                const T vi = vsrc[cell];
+                const int64_t base = a.slice_ptr[slice] + cw * L + l;
+                const int nk = (d > l) ? (d - l + L - 1) / L : 0;
+                int kk = 0;
+                for (; kk + 2 <= nk; kk += 2) {
+                    const int64_t e0 = base + (int64_t)kk * 64, e1 = e0 + 64;
+                    const int j0 = a.col[e0], j1 = a.col[e1];
+                    const T w0 = a.w[e0], w1 = a.w[e1];
+                    const T v0 = vsrc[j0], v1 = vsrc[j1];
+                    acc.add(gj_term(vi, v0, w0));
+                    acc.add(gj_term(vi, v1, w1));
+                }
+                if (kk < nk) {
+                    const int64_t e0 = base + (int64_t)kk * 64;
+                    acc.add(gj_term(vi, vsrc[a.col[e0]], a.w[e0]));
+                }
+            }
+        }
+        if constexpr (L > 1) {
+#pragma unroll
+            for (int off = L / 2; off > 0; off >>= 1) {
+                T os = shfl_xor(acc.s, off);
+                T oc = KAHAN ? shfl_xor(acc.c, off) : T(0);
+                acc.merge(os, oc);
+            }
+        }
+        if (l == 0) ic_lds[local] = acc.value();
+    }
+    __syncthreads();
+
+    // (2) compartment update, one thread per cell
+    const int64_t cell = cell0 + tid;
+    if (cell < a.n) {
+        const int64_t np = a.npad;
+        T *st = a.st;
+        typename CellOf<T>::type c;
+        c.vd = vin[cell];
+        c.r = st[CALCIUM_R * np + cell];
+        c.s = st[POTASSIUM_S * np + cell];
+        c.q = st[HCURRENT_Q * np + cell];
+        c.ca = st[CA2PLUS * np + cell];
+        c.icah = st[I_CAH * np + cell];
+        c.gcal = st[G_CAL * np + cell];
+        c.vs = st[V_SOMA * np + cell];
+        c.h = st[SODIUM_H * np + cell];
+        c.n = st[POTASSIUM_N * np + cell];
+        c.p = st[POTASSIUM_P * np + cell];
+        c.xs = st[POTASSIUM_X_S * np + cell];
+        c.k = st[CALCIUM_K * np + cell];
+        c.l = st[CALCIUM_L * np + cell];
+        c.va = st[V_AXON * np + cell];
+        c.ha = st[SODIUM_H_A * np + cell];
+        c.xa = st[POTASSIUM_X_A * np + cell];
+        c.m = 0; c.ma = 0;
+        ode(a.m, c, ic_lds[tid], iapp_of(a, cell, t));
+        vout[cell] = c.vd;
+        st[CALCIUM_R * np + cell] = c.r;
+        st[POTASSIUM_S * np + cell] = c.s;
+        st[HCURRENT_Q * np + cell] = c.q;
+        st[CA2PLUS * np + cell] = c.ca;
+        st[I_CAH * np + cell] = c.icah;
+        st[V_SOMA * np + cell] = c.vs;
+        st[SODIUM_M * np + cell] = c.m;
+        st[SODIUM_H * np + cell] = c.h;
+        st[POTASSIUM_N * np + cell] = c.n;
+        st[POTASSIUM_P * np + cell] = c.p;
+        st[POTASSIUM_X_S * np + cell] = c.xs;
+        st[CALCIUM_K * np + cell] = c.k;
+        st[CALCIUM_L * np + cell] = c.l;
+        st[V_AXON * np + cell] = c.va;
+        st[SODIUM_M_A * np + cell] = c.ma;
+        st[SODIUM_H_A * np + cell] = c.ha;
+        st[POTASSIUM_X_A * np + cell] = c.xa;
+        if (a.trace && (t % a.stride) == 0)
+            a.trace[((t / a.stride) % a.trace_cap) * a.n + cell] = c.va;
+    }
+    __syncthreads();   // ic_lds / v_lds are reused by the next block or step
+}
+
+/* ---------------------------------------------------------------------- */
+/* kernels                                                                  */
+/* ---------------------------------------------------------------------- */
+
+/* One step: step index = (*d_step if given) + k. */
+template <typename T, int L, bool KAHAN, bool VLDS>
+__global__ __launch_bounds__(kBlock) void k_step(StepArgs<T> a, int64_t k)
+{
+    __shared__ T ic_lds[kBlock];
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
+    const int64_t t = (a.d_step ? *a.d_step : 0) + k;
+    step_block<T, L, KAHAN, VLDS>(a, blockIdx.x, t, ic_lds, reinterpret_cast<T *>(dyn_lds));
+}
+
+/* Bounded grid barrier: every workgroup arrives once per barrier; waits at
+ * most kMaxSpins polls, then records a timeout that makes every later
+ * barrier return at once, so the grid always drains. */
+constexpr unsigned kMaxSpins = 1u << 22;
+
+__device__ __forceinline__ bool grid_barrier(GridBarrier *gb, unsigned target)
+{
+    __shared__ int s_ok;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        if (__hip_atomic_load(&gb->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            ok = 0;
+        } else {
+            __hip_atomic_fetch_add(&gb->count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            unsigned spins = 0;
+            while (__hip_atomic_load(&gb->count, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                if (__hip_atomic_load(&gb->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    ok = 0;
+                    break;
+                }
+                if (++spins > kMaxSpins) {
+                    __hip_atomic_store(&gb->error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        s_ok = ok;
+    }
+    __syncthreads();
+    const int ok = s_ok;
+    __threadfence();
+    return ok != 0;
+}
+
+/* `nsteps` steps starting at *d_step; workgroups loop over cell blocks. */
+template <typename T, int L, bool KAHAN, bool VLDS>
+__global__ __launch_bounds__(kBlock) void k_persist(StepArgs<T> a, GridBarrier *gb, int nsteps)
+{
+    __shared__ T ic_lds[kBlock];
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
+    const int64_t t0 = *a.d_step;
+    const int64_t nblk = a.npad / kBlock;
+    for (int k = 0; k < nsteps; k++) {
+        for (int64_t b = blockIdx.x; b < nblk; b += gridDim.x)
+            step_block<T, L, KAHAN, VLDS>(a, b, t0 + k, ic_lds, reinterpret_cast<T *>(dyn_lds));
+        if (k + 1 < nsteps && !grid_barrier(gb, (unsigned)(k + 1) * gridDim.x)) return;
+    }
+}
+
+/* I_c of every cell from V_dend at step *d_step (no state change). */
+template <typename T, int L, bool KAHAN>
+__global__ __launch_bounds__(kBlock) void k_gap_currents(StepArgs<T> a, int64_t t, T *out)
+{
+    constexpr int CPW = 64 / L;
+    constexpr int CPR = kBlock / L;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int cw = lane / L, l = lane % L;
+    const T *vin = a.vd[t & 1];
+    const int64_t cell0 = (int64_t)blockIdx.x * kBlock;
+    for (int r = 0; r < L; r++) {
+        const int local = r * CPR + wave * CPW + cw;
+        const int64_t cell = cell0 + local;
+        const int64_t slice = (cell0 + r * CPR + wave * CPW) / CPW;
+        Acc<T, KAHAN> acc;
+        if (cell < a.n) {
+            const int d = a.deg[cell];
+            const T vi = vin[cell];
+            const int64_t base = a.slice_ptr[slice] + cw * L + l;
+            for (int kk = 0; kk * L + l < d; kk++) {
+                const int64_t e = base + (int64_t)kk * 64;
+                acc.add(gj_term(vi, vin[a.col[e]], a.w[e]));
+            }
+        }
+        if constexpr (L > 1) {
+            for (int off = L / 2; off > 0; off >>= 1) {
+                T os = shfl_xor(acc.s, off);
+                T oc = KAHAN ? shfl_xor(acc.c, off) : T(0);
+                acc.merge(os, oc);
+            }
+        }
+        if (l == 0 && cell < a.n) out[cell] = acc.value();
+    }
+}
+
+__global__ void k_reset_barrier(GridBarrier *gb)
+{
+    gb->count = 0;
+    gb->error = 0;
+}
+
+__global__ void k_bump_step(int64_t *d_step, int64_t by) { *d_step += by; }
+
+/* flag[0] = 1 if any voltage of the first n cells is NaN/Inf. */
+template <typename T>
+__global__ void k_check_finite(const T *vd, const T *vs, const T *va, int64_t n, int *flag)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        if (!isfinite(vd[i]) || !isfinite(vs[i]) || !isfinite(va[i]))
+            __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__global__ void k_clear_flag(int *flag) { *flag = 0; }
+
+}  // namespace infoli

@@ -1,0 +1,353 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle, the
+reference's golden traces and size-independent properties.
+
+Tolerances (max |dV_axon| in mV, the figure BASELINE states):
+  FP64 GPU vs FP64 reference/oracle: 1e-9 (libm exp differs from the
+      device exp by an ulp; the golden single-cell trace has a spike);
+  FP32 / FP32+compensated GPU vs FP64: as the FP32 oracle itself
+      (2e-3 single cell with spike, 2e-4 10-cell, 2e-2 20-cell with pulses);
+  FP32+compensated vs FP64 on the 100k-cell sparse network over the first
+      100 ms (2000 steps): 1e-3.
+"""
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+import pytest
+
+import infoli_amd as I
+import oracle_lib as O
+from infoli_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+PRECS = {"fp64": O.FP64, "fp32": O.FP32, "fp32k": O.FP32K}
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def make_net(n, prec, rp=None, col=None, w=None, gcal=None, iapp=None, **exec_kw):
+    net = I.Network(n, precision=prec)
+    if rp is not None:
+        net.set_connectivity(rp, col, w)
+    if gcal is not None:
+        net.set_gcal(gcal)
+    if iapp is not None:
+        net.set_iapp(iapp)
+    if exec_kw:
+        net.set_exec(**exec_kw)
+    return net
+
+
+def oracle_ref(prec, n, steps, rp=None, col=None, w=None, gcal=None, iapp=None, **kw):
+    st = O.initial_state(n)
+    if gcal is not None:
+        st[O.G_CAL] = gcal
+    tr, _ = O.run(PRECS[prec], st, steps, rp, col, w, iapp_base=iapp, **kw)
+    return tr, st
+
+
+@pytest.mark.parametrize("case,tol32", [("st_single_cell", 2e-3), ("st_reg10_iapp", 2e-4),
+                                        ("st_rand20", 2e-2)])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "fp32k"])
+def test_reference_golden_traces(golden, case, tol32, prec):
+    say(f"golden {case} {prec}")
+    z = golden(case)
+    conn, iapp, ref = z["conn"], z["iapp"], z["trace"]
+    n = conn.shape[0]
+    rp, col, w = O.dense_to_csr(conn)
+    net = make_net(n, prec, rp, col, w)
+    net.set_iapp_table(0, iapp)
+    out = net.advance(len(iapp), trace_stride=1)
+    err = np.abs(out - ref).max()
+    net.close()
+    assert err < (1e-9 if prec == "fp64" else tol32), err
+
+
+def _random_case(n=700, p=0.08, seed=3):
+    rp, col, w = I.gen_symmetric(seed, n, p, 0.02, 0.06)
+    gcal = I.gen_uniform(seed, 2, n, 0.5, 1.1)
+    iapp = I.gen_uniform(seed, 3, n, -1.0, 3.0)
+    return rp, col, w, gcal, iapp
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-9), ("fp32", 5e-4), ("fp32k", 5e-4)])
+def test_random_network_vs_oracle(prec, tol):
+    say(f"random network {prec}")
+    n, steps = 700, 2000
+    rp, col, w, gcal, iapp = _random_case(n)
+    mask = I.gen_mask(3, 4, n, 0.3)
+    net = make_net(n, prec, rp, col, w, gcal, iapp)
+    net.set_iapp_pulse(6.0, 500, 900, mask)
+    out = net.advance(steps, trace_stride=5)
+    ref, _ = oracle_ref(prec, n, steps, rp, col, w, gcal, iapp, mask=mask, amp=6.0, t0=500,
+                        t1=900, stride=5)
+    ref64, _ = oracle_ref("fp64", n, steps, rp, col, w, gcal, iapp, mask=mask, amp=6.0, t0=500,
+                          t1=900, stride=5)
+    net.close()
+    assert np.abs(out - ref).max() < tol
+    assert np.abs(out - ref64).max() < (1e-9 if prec == "fp64" else 1e-3)
+
+
+MODES = [(I.EXEC_STEP, "step"), (I.EXEC_GRAPH, "graph"), (I.EXEC_PERSISTENT, "persistent")]
+
+
+@pytest.mark.parametrize("mode,mname", MODES)
+@pytest.mark.parametrize("L", [1, 2, 4, 8, 16])
+@pytest.mark.parametrize("vlds", [0, 1])
+@pytest.mark.parametrize("prec", ["fp64", "fp32k"])
+def test_exec_strategies_agree(mode, mname, L, vlds, prec):
+    say(f"exec {mname} L={L} vlds={vlds} {prec}")
+    n, steps = 600, 230
+    rp, col, w, gcal, iapp = _random_case(n, p=0.15, seed=9)
+    net = make_net(n, prec, rp, col, w, gcal, iapp)
+    net.set_exec(mode=mode, lanes_per_cell=L, v_in_lds=vlds, steps_per_launch=50)
+    out = net.advance(steps, trace_stride=10)
+    vd = net.get_state("V_dend")
+    net.close()
+    ref, st = oracle_ref(prec, n, steps, rp, col, w, gcal, iapp, stride=10)
+    tol = 1e-10 if prec == "fp64" else 2e-4
+    assert np.abs(out - ref).max() < tol
+    assert np.abs(vd - st[O.V_DEND]).max() < tol
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "fp32k"])
+@pytest.mark.parametrize("L", [1, 4, 16])
+def test_gap_currents_vs_oracle(prec, L):
+    say(f"gap currents {prec} L={L}")
+    rng = np.random.default_rng(L)
+    n = 1500
+    rp, col, w = I.gen_symmetric(21, n, 0.1, 0.02, 0.06)
+    v = rng.uniform(-70, 10, n)
+    net = make_net(n, prec, rp, col, w, lanes_per_cell=L)
+    net.set_state("V_dend", v)
+    ic = net.gap_currents()
+    net.close()
+    ref = O.gap_currents(O.FP64, v, rp, col, w)
+    scale = np.abs(ref).max()
+    tol = 1e-13 if prec == "fp64" else (2e-6 if prec == "fp32" else 4e-7)
+    assert np.abs(ic - ref).max() <= tol * scale * (1 if prec == "fp64" else 10)
+    if prec == "fp64":
+        assert abs(ic.sum()) <= 1e-12 * np.abs(ic).sum()   # symmetric w conserves current
+
+
+def test_trace_split_calls_equal_single_call():
+    say("trace split")
+    n = 300
+    rp, col, w, gcal, iapp = _random_case(n, seed=4)
+    a = make_net(n, "fp32k", rp, col, w, gcal, iapp)
+    b = make_net(n, "fp32k", rp, col, w, gcal, iapp)
+    ta = a.advance(101, trace_stride=7)
+    tb = np.concatenate([b.advance(k, trace_stride=7) for k in (33, 33, 35)])
+    assert a.step_count == b.step_count == 101
+    assert ta.shape == tb.shape == (15, n)
+    assert (ta == tb).all()
+    a.close(); b.close()
+
+
+def test_state_roundtrip_and_iapp_table():
+    say("state roundtrip")
+    n = 257
+    rp, col, w, gcal, iapp = _random_case(n, seed=5)
+    rng = np.random.default_rng(0)
+    net = make_net(n, "fp64", rp, col, w, gcal)
+    for k, name in enumerate(I.VARS):
+        vals = rng.uniform(0, 1, n) if name[0] != "V" else rng.uniform(-70, -50, n)
+        net.set_state(name, vals)
+        assert (net.get_state(k) == vals).all()
+    net.close()
+    # a table holding base + pulse gives the same bits as base + pulse
+    steps = 300
+    mask = I.gen_mask(1, 4, n, 0.5)
+    a = make_net(n, "fp32k", rp, col, w, gcal, iapp)
+    a.set_iapp_pulse(6.0, 100, 200, mask)
+    ta = a.advance(steps, trace_stride=1)
+    tab = np.tile(iapp.astype(np.float32).astype(np.float64), (steps, 1))
+    pulsed = (np.float32(iapp) + np.float32(6.0)).astype(np.float64)
+    tab[100:200, mask == 1] = pulsed[mask == 1]
+    b = make_net(n, "fp32k", rp, col, w, gcal)
+    b.set_iapp_table(0, tab)
+    tb = b.advance(steps, trace_stride=1)
+    assert (ta == tb).all()
+    a.close(); b.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32k"])
+def test_edge_shapes(prec):
+    say(f"edge shapes {prec}")
+    # one cell, no neighbours
+    net = make_net(1, prec)
+    out = net.advance(50, trace_stride=1)
+    ref, _ = oracle_ref(prec, 1, 50)
+    assert np.abs(out - ref).max() < (1e-12 if prec == "fp64" else 1e-5)
+    net.close()
+    # star: a hub with 1100 spokes (max degree far above the mean), spokes
+    # with distinct drive
+    n = 1101
+    rows = [np.arange(1, n)] + [np.array([0])] * (n - 1)
+    rp = np.zeros(n + 1, np.int64)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    col = np.concatenate(rows).astype(np.int32)
+    w = np.full(len(col), 0.05)
+    iapp = np.linspace(-1, 3, n)
+    for L in (1, 16):
+        net = make_net(n, prec, rp, col, w, iapp=iapp, lanes_per_cell=L)
+        out = net.advance(400, trace_stride=20)
+        ref, _ = oracle_ref(prec, n, 400, rp, col, w, iapp=iapp, stride=20)
+        assert np.abs(out - ref).max() < (1e-9 if prec == "fp64" else 5e-4), L
+        net.close()
+    # an empty graph of many cells equals the single-cell trajectory
+    n = 1000
+    net = make_net(n, prec)
+    out = net.advance(100, trace_stride=1)
+    one, _ = oracle_ref(prec, 1, 100)
+    assert np.abs(out - one).max() < (1e-12 if prec == "fp64" else 1e-5)
+    net.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "fp32k"])
+def test_synchrony_bitwise(prec):
+    say(f"synchrony {prec}")
+    n = 2000
+    rp, col, w = I.gen_symmetric(8, n, 0.05, 0.02, 0.06)
+    net = make_net(n, prec, rp, col, w)
+    net.set_iapp_pulse(6.0, 100, 300)
+    out = net.advance(800, trace_stride=1)
+    one = make_net(1, prec)
+    one.set_iapp_pulse(6.0, 100, 300)
+    ref = one.advance(800, trace_stride=1)
+    assert (out == ref).all()
+    net.close(); one.close()
+
+
+def test_nonfinite_watchdog():
+    say("watchdog")
+    net = make_net(4, "fp64")
+    v = np.full(4, -60.0)
+    v[2] = -8.5   # 0/0 in beta_r, as in the reference
+    net.set_state("V_dend", v)
+    with pytest.raises(I.InfoliError) as e:
+        net.advance(1)
+    assert e.value.code == -5
+    assert np.isnan(net.get_state("Calcium_r")[2])
+    assert np.isfinite(net.get_state("Calcium_r")[[0, 1, 3]]).all()
+    net.close()
+
+
+def test_partition_batched_equals_separate():
+    say("partition")
+    wl = W.batched_512(n_nets=6, lo=64, hi=700, steps=300)
+    net = I.Network(wl["n"], precision="fp64")
+    W.load_into(net, wl)
+    out = net.advance(300, trace_stride=10)
+    net.close()
+    offs = wl["partition"]
+    for k in range(6):
+        a, b = offs[k], offs[k + 1]
+        rp = wl["row_ptr"][a:b + 1] - wl["row_ptr"][a]
+        col = wl["col"][wl["row_ptr"][a]:wl["row_ptr"][b]] - a
+        w = wl["w"][wl["row_ptr"][a]:wl["row_ptr"][b]]
+        sub = make_net(b - a, "fp64", rp, col, w, wl["gcal"][a:b], wl["iapp_base"][a:b])
+        o = sub.advance(300, trace_stride=10)
+        sub.close()
+        assert np.abs(o - out[:, a:b]).max() < 1e-11
+
+
+def test_autotune_restores_state():
+    say("autotune")
+    n = 5000
+    rp, col, w, gcal, iapp = _random_case(n, p=0.02, seed=12)
+    a = make_net(n, "fp32k", rp, col, w, gcal, iapp)
+    a.advance(37)
+    before = a.state()
+    chosen = a.autotune(trial_steps=20)
+    assert (a.state() == before).all() and a.step_count == 37
+    assert chosen["mode"] in (I.EXEC_GRAPH, I.EXEC_PERSISTENT)
+    out = a.advance(100, trace_stride=10)
+    b = make_net(n, "fp32k", rp, col, w, gcal, iapp)
+    b.advance(37)
+    ref = b.advance(100, trace_stride=10)
+    assert np.abs(out - ref).max() < 1e-4
+    a.close(); b.close()
+
+
+def test_sparse_100k_full_size_precision():
+    """The 100k-cell sparse configuration at full size: one step equals the
+    FP64 oracle; over the first 100 ms (2000 steps) FP32+compensated stays
+    within 1e-3 mV of FP64 on the GPU."""
+    say("sparse_100k full size")
+    wl = W.sparse_100k()
+    n = wl["n"]
+    nets = {}
+    for prec in ("fp64", "fp32k"):
+        net = I.Network(n, precision=prec)
+        W.load_into(net, wl)
+        nets[prec] = net
+    one64 = nets["fp64"].advance(1, trace_stride=1)
+    st = O.initial_state(n)
+    st[O.G_CAL] = wl["gcal"]
+    ref, _ = O.run(O.FP64, st, 1, wl["row_ptr"], wl["col"], wl["w"], mask=wl["mask"], amp=6.0,
+                   t0=2000, t1=4000)
+    assert np.abs(one64 - ref).max() < 1e-12
+    ic = nets["fp64"].gap_currents()
+    ic_ref = O.gap_currents(O.FP64, nets["fp64"].get_state("V_dend"), wl["row_ptr"], wl["col"],
+                            wl["w"])
+    assert np.abs(ic - ic_ref).max() <= 1e-12 * max(np.abs(ic_ref).max(), 1e-30)
+    nets["fp32k"].advance(1)
+    t64 = nets["fp64"].advance(1999, trace_stride=1)
+    t32 = nets["fp32k"].advance(1999, trace_stride=1)
+    err = np.abs(t64 - t32).max()
+    say(f"  max |dV| FP32+compensated vs FP64 over 100 ms: {err:.3e} mV")
+    for net in nets.values():
+        net.close()
+    assert err <= 1e-3
+
+
+def test_driver_reproduces_regression_baseline(golden):
+    say("driver regression")
+    exe = os.path.join(O.ROOT, "infoli_amd", "bin", "infoli.x")
+    with tempfile.TemporaryDirectory() as d:
+        shutil.copy(os.path.join(O.GOLDEN, "gcal_file.txt"), d)
+        r = subprocess.run([exe, "-n", "10", "-p", "1", "-t", "500", "-g", "gcal_file.txt",
+                            "-c", os.path.join(O.GOLDEN, "default.conf")], cwd=d,
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        assert "Execution Time for Simulation" in r.stdout
+        out = np.loadtxt(os.path.join(d, "InferiorOlive_Output.txt"))
+        z = golden("stable_reg10")
+        assert out.shape == (10000, 10)
+        base = z["trace"].astype(np.float64)
+        test = out[z["step_index"]]
+        rel = np.max(np.abs((base - test) / base))
+        assert rel <= 0.01 and np.abs(base - test).max() < 5e-3
+        checker = os.path.join(O.REF_DIR, "checker.x")
+        if os.path.exists(checker):
+            with open(os.path.join(d, "InferiorOlive_Output_Baseline.txt"), "w") as fh:
+                for row in z["trace"]:
+                    fh.write("".join("%.8f " % x for x in row) + "\n")
+            with open(os.path.join(d, "InferiorOlive_Output.txt"), "w") as fh:
+                for row in test:
+                    fh.write("".join("%.8f " % x for x in row) + "\n")
+            r = subprocess.run([checker, "10"], cwd=d, capture_output=True, text=True)
+            assert "Check was passed successfully" in r.stdout, r.stdout
+
+
+def test_driver_reference_format():
+    say("driver format")
+    exe = os.path.join(O.ROOT, "infoli_amd", "bin", "infoli.x")
+    with tempfile.TemporaryDirectory() as d:
+        r = subprocess.run([exe, "-n", "5", "-p", "1", "-t", "20", "-s", "1"], cwd=d,
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout
+        lines = open(os.path.join(d, "InferiorOlive_Output.txt")).read().splitlines()
+    assert lines[0].startswith("Execution Time for Simulation in ms: ")
+    assert lines[1] == "#simSteps Input(Iapp) Output(V_axon)0 1 2 3 4 "
+    assert len(lines) == 2 + 4            # 400 steps, a record every 100
+    tok = lines[2].split()
+    assert tok[:2] == ["1", "0.00"] and tok[2:5] == ["1", ":", "-59.97596741"]
