@@ -181,12 +181,19 @@ int infoli_gap_currents(infoli_t *ctx, double *i_c);
 #define INFOLI_EXEC_PERSISTENT  3  /* multi-step kernel with grid barrier    */
 #define INFOLI_EXEC_BATCHED     4  /* per-network workgroups, LDS-resident V */
 
+/* How a step reads the neighbours' V_dend. */
+#define INFOLI_GATHER_DIRECT    0  /* random loads from the V_dend array        */
+#define INFOLI_GATHER_LDS       1  /* whole V_dend array staged in LDS (small n) */
+#define INFOLI_GATHER_STAGED    2  /* per cell block: senders sorted, copied to
+                                      LDS, then read in each row's order        */
+
 typedef struct infoli_exec {
     int     mode;            /* INFOLI_EXEC_*                                 */
     int     lanes_per_cell;  /* 1,2,4,8,16: lanes sharing one neighbour list  */
-    int     block;           /* threads per workgroup (64..1024)             */
+    int     block;           /* threads per workgroup (256)                   */
     int     steps_per_launch;/* graph length / persistent steps per launch    */
-    int     v_in_lds;        /* stage V_dend in LDS for the gather (0/1)      */
+    int     gather;          /* INFOLI_GATHER_*                               */
+    int     lds_kb;          /* staged gather: LDS budget per workgroup, KiB  */
 } infoli_exec;
 
 int infoli_get_exec(const infoli_t *ctx, infoli_exec *out);

@@ -14,6 +14,9 @@ from .network import (  # noqa: F401
     EXEC_GRAPH,
     EXEC_PERSISTENT,
     EXEC_STEP,
+    GATHER_DIRECT,
+    GATHER_LDS,
+    GATHER_STAGED,
     VARS,
     InfoliError,
     Network,

@@ -39,11 +39,12 @@ namespace {
 
 constexpr size_t kTraceRingBytes = size_t(1) << 30;   // device trace ring budget
 constexpr size_t kVldsMaxBytes = 96 * 1024;           // V_dend staged in LDS up to this size
+constexpr int kDefaultLdsKb = 76;                     // staged gather: two workgroups per CU
 
 int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
 struct GraphKey {
-    int mode = -1, L = 0, vlds = 0, G = 0, grid = 0;
+    int mode = -1, L = 0, gather = 0, G = 0, grid = 0;
     size_t args_len = 0;
     unsigned char args[sizeof(StepArgs<double>)];
 };
@@ -66,6 +67,16 @@ struct infoli_ctx {
     void *d_w = nullptr;
     int sell_L = 0;                 // lanes per cell of the layout on the device
     int64_t sell_entries = 0;
+
+    // staged-gather layout (built for one lanes-per-cell value and LDS budget)
+    bool staged_valid = false;
+    int staged_kb = 0;
+    int64_t st_nblocks = 0, st_elds = 0;
+    int st_sb = 0;
+    int32_t *d_blk_cell0 = nullptr, *d_blk_ncell = nullptr, *d_lrs = nullptr;
+    int64_t *d_blk_gbase = nullptr;
+    uint32_t *d_gl32 = nullptr;
+    uint64_t *d_gl64 = nullptr;
 
     void *d_iapp_base = nullptr;
     uint8_t *d_mask = nullptr;
@@ -172,6 +183,18 @@ StepArgs<T> make_args(const infoli_ctx *c, int64_t stride)
     a.stride = stride > 0 ? stride : 1;
     a.trace_cap = c->trace_cap > 0 ? c->trace_cap : 1;
     a.d_step = nullptr;
+    if (c->exec.gather == GATHER_STAGED) {
+        a.nblocks = c->st_nblocks;
+        a.blk_cell0 = c->d_blk_cell0;
+        a.blk_ncell = c->d_blk_ncell;
+        a.blk_gbase = c->d_blk_gbase;
+        a.lrs = c->d_lrs;
+        a.gl32 = c->d_gl32;
+        a.gl64 = c->d_gl64;
+        a.sb = c->st_sb;
+    } else {
+        a.nblocks = c->npad / kBlock;
+    }
     a.m = make_model<T>(c->params);
     return a;
 }
@@ -200,16 +223,19 @@ int lidx(int L)
      (const void *)KERN<T, 4, __VA_ARGS__>, (const void *)KERN<T, 8, __VA_ARGS__>, \
      (const void *)KERN<T, 16, __VA_ARGS__>}
 
-/* [precision][vlds][L] */
-const void *const k_step_tab[3][2][5] = {
-    {INFOLI_L_TABLE(k_step, double, false, false), INFOLI_L_TABLE(k_step, double, false, true)},
-    {INFOLI_L_TABLE(k_step, float, false, false), INFOLI_L_TABLE(k_step, float, false, true)},
-    {INFOLI_L_TABLE(k_step, float, true, false), INFOLI_L_TABLE(k_step, float, true, true)},
+/* [precision][gather][L] */
+#define INFOLI_G_TABLE(KERN, T, K) \
+    {INFOLI_L_TABLE(KERN, T, K, GATHER_DIRECT), INFOLI_L_TABLE(KERN, T, K, GATHER_VLDS), \
+     INFOLI_L_TABLE(KERN, T, K, GATHER_STAGED)}
+const void *const k_step_tab[3][3][5] = {
+    INFOLI_G_TABLE(k_step, double, false),
+    INFOLI_G_TABLE(k_step, float, false),
+    INFOLI_G_TABLE(k_step, float, true),
 };
-const void *const k_persist_tab[3][2][5] = {
-    {INFOLI_L_TABLE(k_persist, double, false, false), INFOLI_L_TABLE(k_persist, double, false, true)},
-    {INFOLI_L_TABLE(k_persist, float, false, false), INFOLI_L_TABLE(k_persist, float, false, true)},
-    {INFOLI_L_TABLE(k_persist, float, true, false), INFOLI_L_TABLE(k_persist, float, true, true)},
+const void *const k_persist_tab[3][3][5] = {
+    INFOLI_G_TABLE(k_persist, double, false),
+    INFOLI_G_TABLE(k_persist, float, false),
+    INFOLI_G_TABLE(k_persist, float, true),
 };
 const void *const k_gap_tab[3][5] = {
     INFOLI_L_TABLE(k_gap_currents, double, false),
@@ -219,38 +245,59 @@ const void *const k_gap_tab[3][5] = {
 
 bool is_f32(const infoli_ctx *c) { return c->prec != INFOLI_FP64; }
 
-size_t vlds_bytes(const infoli_ctx *c) { return c->exec.v_in_lds ? (size_t)c->n * c->tsize : 0; }
+size_t dyn_lds_bytes(const infoli_ctx *c)
+{
+    switch (c->exec.gather) {
+    case GATHER_VLDS: return (size_t)c->n * c->tsize;
+    case GATHER_STAGED: return (size_t)std::max<int64_t>(c->st_elds, 1) * c->tsize;
+    default: return 0;
+    }
+}
+
+/* raise the dynamic-LDS limit of a kernel above 64 KiB when needed */
+void allow_dyn_lds(const void *fn, size_t bytes)
+{
+    if (bytes > 64 * 1024)
+        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+int64_t n_blocks(const infoli_ctx *c)
+{
+    return c->exec.gather == GATHER_STAGED ? c->st_nblocks : c->npad / kBlock;
+}
 
 int launch_step(infoli_ctx *c, const void *args, int64_t k)
 {
-    const void *fn = k_step_tab[c->prec][c->exec.v_in_lds ? 1 : 0][lidx(c->exec.lanes_per_cell)];
-    const int64_t nblk = c->npad / kBlock;
+    const void *fn = k_step_tab[c->prec][c->exec.gather][lidx(c->exec.lanes_per_cell)];
+    const int64_t nblk = n_blocks(c);
     int64_t kk = k;
     void *kargs[] = {const_cast<void *>(args), &kk};
-    HIPCHK(hipLaunchKernel(fn, dim3((unsigned)nblk), dim3(kBlock), kargs, vlds_bytes(c), c->stream));
+    allow_dyn_lds(fn, dyn_lds_bytes(c));
+    HIPCHK(hipLaunchKernel(fn, dim3((unsigned)nblk), dim3(kBlock), kargs, dyn_lds_bytes(c), c->stream));
     return INFOLI_OK;
 }
 
 int persist_grid(infoli_ctx *c)
 {
-    const void *fn = k_persist_tab[c->prec][c->exec.v_in_lds ? 1 : 0][lidx(c->exec.lanes_per_cell)];
+    const void *fn = k_persist_tab[c->prec][c->exec.gather][lidx(c->exec.lanes_per_cell)];
     int per_cu = 0;
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock, vlds_bytes(c)));
+    allow_dyn_lds(fn, dyn_lds_bytes(c));
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock, dyn_lds_bytes(c)));
     if (per_cu < 1) return infoli_fail(INFOLI_ERR_HIP, "persistent kernel does not fit on a CU");
-    const int64_t nblk = c->npad / kBlock;
+    const int64_t nblk = n_blocks(c);
     int64_t g = (int64_t)per_cu * c->num_cus;
     return (int)std::min<int64_t>(g, nblk);
 }
 
 int launch_persist(infoli_ctx *c, const void *args, int nsteps, int grid)
 {
-    const void *fn = k_persist_tab[c->prec][c->exec.v_in_lds ? 1 : 0][lidx(c->exec.lanes_per_cell)];
+    const void *fn = k_persist_tab[c->prec][c->exec.gather][lidx(c->exec.lanes_per_cell)];
     GridBarrier *gb = c->d_bar;
     int ns = nsteps;
     void *kargs[] = {const_cast<void *>(args), &gb, &ns};
     hipLaunchKernelGGL(k_reset_barrier, dim3(1), dim3(1), 0, c->stream, c->d_bar);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipLaunchKernel(fn, dim3((unsigned)grid), dim3(kBlock), kargs, vlds_bytes(c), c->stream));
+    HIPCHK(hipLaunchKernel(fn, dim3((unsigned)grid), dim3(kBlock), kargs, dyn_lds_bytes(c), c->stream));
     return INFOLI_OK;
 }
 
@@ -300,13 +347,123 @@ int upload_sell(infoli_ctx *c, int L)
     return INFOLI_OK;
 }
 
+int bits_for(int64_t v)   // bits to hold values 0..v
+{
+    int b = 1;
+    while (b < 63 && (v >> b) != 0) b++;
+    return b;
+}
+
+void free_staged(infoli_ctx *c)
+{
+    void *bufs[] = {c->d_blk_cell0, c->d_blk_ncell, c->d_lrs, c->d_blk_gbase, c->d_gl32, c->d_gl64};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    c->d_blk_cell0 = c->d_blk_ncell = c->d_lrs = nullptr;
+    c->d_blk_gbase = nullptr;
+    c->d_gl32 = nullptr;
+    c->d_gl64 = nullptr;
+    c->staged_valid = false;
+    c->staged_kb = 0;
+    c->st_nblocks = c->st_elds = 0;
+}
+
+/* Staged-gather layout: cell blocks of 64..256 cells (multiples of 64, so
+ * sliced-ELL slices never straddle a block) whose neighbour entries fit
+ * the LDS budget; per block the list of (sender, LDS slot) sorted by
+ * sender, slot = the entry's position in the block's compact row-major
+ * list; and per cell its LDS row start. */
+int upload_staged(infoli_ctx *c, int kb)
+{
+    const int64_t n = c->n, npad = c->npad;
+    const int64_t emax = (int64_t)kb * 1024 / (int64_t)c->tsize;
+    auto deg_of = [&](int64_t i) -> int64_t { return i < n ? c->row_ptr[i + 1] - c->row_ptr[i] : 0; };
+    std::vector<int32_t> b0, bn;
+    std::vector<int64_t> bent;
+    for (int64_t g = 0; g < npad; g += 64) {
+        int64_t ge = 0;
+        for (int64_t i = g; i < g + 64; i++) ge += deg_of(i);
+        if (ge > emax)
+            return infoli_fail(INFOLI_ERR_ARG,
+                               "staged gather: cells %lld..%lld have %lld neighbour entries, more than "
+                               "the %d KiB LDS budget holds (%lld)",
+                               (long long)g, (long long)(g + 63), (long long)ge, kb, (long long)emax);
+        if (!b0.empty() && bn.back() + 64 <= kBlock && bent.back() + ge <= emax) {
+            bn.back() += 64;
+            bent.back() += ge;
+        } else {
+            b0.push_back((int32_t)g);
+            bn.push_back(64);
+            bent.push_back(ge);
+        }
+    }
+    const int64_t nb = (int64_t)b0.size();
+    int64_t elds = 1;
+    for (int64_t e : bent) elds = std::max(elds, e);
+    std::vector<int32_t> lrs(npad, 0);
+    std::vector<int64_t> gbase(nb + 1, 0);
+    for (int64_t b = 0; b < nb; b++) {
+        int64_t run = 0;
+        for (int64_t i = b0[b]; i < b0[b] + bn[b]; i++) {
+            lrs[i] = (int32_t)run;
+            run += deg_of(i);
+        }
+        gbase[b + 1] = gbase[b] + run;
+    }
+    const int sb = bits_for(std::max<int64_t>(elds - 1, 1));
+    const int cb = bits_for(std::max<int64_t>(n - 1, 1));
+    const bool packed = sb + cb <= 32;
+    const int64_t ne = gbase[nb];
+    std::vector<uint64_t> list(std::max<int64_t>(ne, 1), 0);
+    for (int64_t b = 0; b < nb; b++) {
+        uint64_t *lb = list.data() + gbase[b];
+        int64_t k = 0;
+        for (int64_t i = b0[b]; i < b0[b] + bn[b] && i < n; i++)
+            for (int64_t e = c->row_ptr[i]; e < c->row_ptr[i + 1]; e++)
+                lb[k++] = ((uint64_t)(uint32_t)c->col[e] << 32) | (uint64_t)(lrs[i] + (e - c->row_ptr[i]));
+        std::sort(lb, lb + k);
+    }
+    free_staged(c);
+    HIPCHK(hipMalloc(&c->d_blk_cell0, sizeof(int32_t) * nb));
+    HIPCHK(hipMalloc(&c->d_blk_ncell, sizeof(int32_t) * nb));
+    HIPCHK(hipMalloc(&c->d_blk_gbase, sizeof(int64_t) * (nb + 1)));
+    HIPCHK(hipMalloc(&c->d_lrs, sizeof(int32_t) * npad));
+    HIPCHK(hipMemcpy(c->d_blk_cell0, b0.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_blk_ncell, bn.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_blk_gbase, gbase.data(), sizeof(int64_t) * (nb + 1), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_lrs, lrs.data(), sizeof(int32_t) * npad, hipMemcpyHostToDevice));
+    if (packed) {
+        std::vector<uint32_t> l32(list.size());
+        for (size_t k = 0; k < list.size(); k++)
+            l32[k] = ((uint32_t)(list[k] >> 32) << sb) | (uint32_t)(list[k] & 0xFFFFFFFFu);
+        HIPCHK(hipMalloc(&c->d_gl32, sizeof(uint32_t) * l32.size()));
+        HIPCHK(hipMemcpy(c->d_gl32, l32.data(), sizeof(uint32_t) * l32.size(), hipMemcpyHostToDevice));
+    } else {
+        HIPCHK(hipMalloc(&c->d_gl64, sizeof(uint64_t) * list.size()));
+        HIPCHK(hipMemcpy(c->d_gl64, list.data(), sizeof(uint64_t) * list.size(), hipMemcpyHostToDevice));
+    }
+    c->st_nblocks = nb;
+    c->st_elds = elds;
+    c->st_sb = sb;
+    c->staged_valid = true;
+    c->staged_kb = kb;
+    return INFOLI_OK;
+}
+
 int ensure_layout(infoli_ctx *c)
 {
-    if (c->sell_L == c->exec.lanes_per_cell) return INFOLI_OK;
-    HIPCHK(hipSetDevice(c->device));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    return is_f32(c) ? upload_sell<float>(c, c->exec.lanes_per_cell)
-                     : upload_sell<double>(c, c->exec.lanes_per_cell);
+    const int L = c->exec.lanes_per_cell;
+    if (c->sell_L != L) {
+        HIPCHK(hipSetDevice(c->device));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        RCHK(is_f32(c) ? upload_sell<float>(c, L) : upload_sell<double>(c, L));
+    }
+    if (c->exec.gather == GATHER_STAGED && (!c->staged_valid || c->staged_kb != c->exec.lds_kb)) {
+        HIPCHK(hipSetDevice(c->device));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        RCHK(upload_staged(c, c->exec.lds_kb));
+    }
+    return INFOLI_OK;
 }
 
 void drop_graph(infoli_ctx *c)
@@ -327,6 +484,17 @@ int choose_lanes(double mean_deg)
     return 16;
 }
 
+/* every aligned group of 64 cells fits the staged-gather LDS budget */
+bool max_block_entries_fit(const infoli_ctx *c, int kb)
+{
+    const int64_t emax = (int64_t)kb * 1024 / (int64_t)c->tsize;
+    for (int64_t g = 0; g < c->n; g += 64) {
+        const int64_t hi = std::min<int64_t>(g + 64, c->n);
+        if (c->row_ptr[hi] - c->row_ptr[g] > emax) return false;
+    }
+    return true;
+}
+
 void default_exec(infoli_ctx *c)
 {
     const double mean_deg = c->n > 0 ? (double)c->col.size() / (double)c->n : 0;
@@ -334,7 +502,13 @@ void default_exec(infoli_ctx *c)
     c->exec.lanes_per_cell = choose_lanes(mean_deg);
     c->exec.block = kBlock;
     c->exec.steps_per_launch = 100;
-    c->exec.v_in_lds = ((size_t)c->n * c->tsize <= kVldsMaxBytes && mean_deg >= 64) ? 1 : 0;
+    c->exec.lds_kb = kDefaultLdsKb;
+    if ((size_t)c->n * c->tsize <= kVldsMaxBytes && mean_deg >= 64)
+        c->exec.gather = GATHER_VLDS;
+    else if (mean_deg >= 4 && max_block_entries_fit(c, c->exec.lds_kb))
+        c->exec.gather = GATHER_STAGED;
+    else
+        c->exec.gather = GATHER_DIRECT;
 }
 
 int ensure_stage(infoli_ctx *c, size_t bytes)
@@ -477,6 +651,7 @@ extern "C" void infoli_destroy(infoli_t *c)
                     c->d_iapp_base, c->d_mask, c->d_tab, c->d_trace, c->d_step, c->d_bar, c->d_flag};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
+    free_staged(c);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -524,6 +699,7 @@ extern "C" int infoli_set_connectivity_csr(infoli_t *c, const int64_t *row_ptr,
     c->col.assign(col ? col : nullptr, col ? col + nnz : nullptr);
     c->w.assign(w ? w : nullptr, w ? w + nnz : nullptr);
     c->max_deg = max_deg;
+    c->staged_valid = false;
     const int keep_mode = c->exec.mode;
     default_exec(c);
     if (keep_mode != INFOLI_EXEC_GRAPH) c->exec.mode = keep_mode;
@@ -672,11 +848,21 @@ extern "C" int infoli_set_exec(infoli_t *c, const infoli_exec *e)
     if (x.block != kBlock) return infoli_fail(INFOLI_ERR_ARG, "block must be %d", kBlock);
     if (x.steps_per_launch < 1 || x.steps_per_launch > 100000)
         return infoli_fail(INFOLI_ERR_ARG, "steps_per_launch %d out of range", x.steps_per_launch);
-    if (x.v_in_lds && (size_t)c->n * c->tsize > kVldsMaxBytes)
-        return infoli_fail(INFOLI_ERR_ARG, "v_in_lds: %lld cells do not fit the LDS budget", (long long)c->n);
+    if (x.gather < GATHER_DIRECT || x.gather > GATHER_STAGED)
+        return infoli_fail(INFOLI_ERR_ARG, "infoli_set_exec: gather %d unknown", x.gather);
+    if (x.gather == GATHER_VLDS && (size_t)c->n * c->tsize > kVldsMaxBytes)
+        return infoli_fail(INFOLI_ERR_ARG, "gather LDS: %lld cells do not fit the LDS budget", (long long)c->n);
+    if (x.lds_kb < 8 || x.lds_kb > 156)
+        return infoli_fail(INFOLI_ERR_ARG, "lds_kb %d outside [8, 156]", x.lds_kb);
+    if (x.gather == GATHER_STAGED && !max_block_entries_fit(c, x.lds_kb))
+        return infoli_fail(INFOLI_ERR_ARG, "staged gather: a 64-cell group exceeds the %d KiB LDS budget",
+                           x.lds_kb);
+    const infoli_exec prev = c->exec;
     c->exec = x;
     drop_graph(c);
-    return ensure_layout(c);
+    int rc = ensure_layout(c);
+    if (rc != INFOLI_OK) c->exec = prev;
+    return rc;
 }
 
 extern "C" int infoli_set_profiling(infoli_t *c, int on)
@@ -722,12 +908,12 @@ int capture_graph(infoli_ctx *c, const StepArgs<T> &a, int mode, int G, int grid
     GraphKey key;
     key.mode = mode;
     key.L = c->exec.lanes_per_cell;
-    key.vlds = c->exec.v_in_lds;
+    key.gather = c->exec.gather;
     key.G = G;
     key.grid = grid;
     key.args_len = sizeof a;
     memcpy(key.args, &a, sizeof a);
-    if (c->gexec && c->gkey.mode == key.mode && c->gkey.L == key.L && c->gkey.vlds == key.vlds &&
+    if (c->gexec && c->gkey.mode == key.mode && c->gkey.L == key.L && c->gkey.gather == key.gather &&
         c->gkey.G == key.G && c->gkey.grid == key.grid && c->gkey.args_len == key.args_len &&
         memcmp(c->gkey.args, key.args, key.args_len) == 0)
         return INFOLI_OK;
@@ -980,13 +1166,17 @@ extern "C" int infoli_autotune(infoli_t *c, int64_t trial_steps, infoli_exec *ch
     std::vector<infoli_exec> cands;
     const int Ls[] = {1, 2, 4, 8, 16};
     const int modes[] = {INFOLI_EXEC_GRAPH, INFOLI_EXEC_PERSISTENT};
-    for (int mode : modes)
-        for (int L : Ls)
-            for (int vl = 0; vl < 2; vl++) {
-                if (vl && (size_t)c->n * c->tsize > kVldsMaxBytes) continue;
-                infoli_exec e{mode, L, kBlock, (int)std::min<int64_t>(trial_steps, 500), vl};
-                cands.push_back(e);
-            }
+    const int kbs[] = {40, 76, 150};
+    const int spl = (int)std::min<int64_t>(trial_steps, 500);
+    for (int L : Ls)
+        for (int mode : modes) {
+            cands.push_back(infoli_exec{mode, L, kBlock, spl, GATHER_DIRECT, kDefaultLdsKb});
+            if ((size_t)c->n * c->tsize <= kVldsMaxBytes)
+                cands.push_back(infoli_exec{mode, L, kBlock, spl, GATHER_VLDS, kDefaultLdsKb});
+            for (int kb : kbs)
+                if (max_block_entries_fit(c, kb))
+                    cands.push_back(infoli_exec{mode, L, kBlock, spl, GATHER_STAGED, kb});
+        }
     double best = 1e300;
     infoli_exec best_e = orig;
     int rc = INFOLI_OK;

@@ -56,7 +56,24 @@ struct StepArgs {
     T *trace;             // ring of trace_cap records of n values (nullable)
     int64_t stride, trace_cap;
     const int64_t *d_step;// device step counter (nullable: step index is absolute)
+    // cell blocks: uniform 256-cell blocks (direct / whole-V-in-LDS gathers)
+    // or the staged-gather table
+    int64_t nblocks;
+    const int32_t *blk_cell0;  // first cell of each block (multiple of 64)
+    const int32_t *blk_ncell;  // cells per block (multiple of 64, <= 256)
+    const int64_t *blk_gbase;  // gather-list start of each block, nblocks + 1
+    const int32_t *lrs;        // LDS row start of each cell within its block
+    const uint32_t *gl32;      // (sender << sb) | slot, sorted by sender within a block
+    const uint64_t *gl64;      // (sender << 32) | slot, when the packed form does not fit
+    int sb;                    // slot bits of gl32
     Model<T> m;
+};
+
+/* how a step reads its neighbours' V_dend */
+enum Gather {
+    GATHER_DIRECT = 0,   // random loads from the V_dend array (L2-resident)
+    GATHER_VLDS = 1,     // the whole V_dend array staged in LDS (small networks)
+    GATHER_STAGED = 2    // per block: senders sorted, copied to LDS, then read in row order
 };
 
 struct GridBarrier {
@@ -349,7 +366,7 @@ __device__ __forceinline__ float shfl_xor(float v, int m) { return __shfl_xor(v,
 __device__ __forceinline__ double shfl_xor(double v, int m) { return __shfl_xor(v, m, 64); }
 
 /* ---------------------------------------------------------------------- */
-/* one workgroup, one step                                                  */
+/* one workgroup, one block of cells, one step                              */
 /* ---------------------------------------------------------------------- */
 
 template <typename T>
@@ -362,50 +379,109 @@ __device__ __forceinline__ T iapp_of(const StepArgs<T> &a, int64_t cell, int64_t
     return on ? b + a.amp : b;
 }
 
-template <typename T, int L, bool KAHAN, bool VLDS>
+/* Phase 1 of the staged gather: the block's neighbour entries, sorted by
+ * sender id, copy V_dend[sender] into LDS slot `slot` (the entry's position
+ * in the block's compact row-major list).  Sorted senders make the 64 lanes
+ * of a load touch a few cache lines instead of 64. */
+template <typename T>
+__device__ __forceinline__ void stage_gather(const StepArgs<T> &a, int64_t blk, const T *__restrict__ vin,
+                                             T *__restrict__ vj)
+{
+    const int tid = threadIdx.x;
+    const int64_t g0 = a.blk_gbase[blk], g1 = a.blk_gbase[blk + 1];
+    if (a.gl32) {
+        const uint32_t *__restrict__ gl = a.gl32;
+        const unsigned sb = (unsigned)a.sb;
+        const uint32_t mask = (1u << sb) - 1u;
+        int64_t e = g0 + tid;
+        for (; e + 3 * kBlock < g1; e += 4 * kBlock) {
+            const uint32_t u0 = gl[e], u1 = gl[e + kBlock], u2 = gl[e + 2 * kBlock], u3 = gl[e + 3 * kBlock];
+            const T x0 = vin[u0 >> sb], x1 = vin[u1 >> sb], x2 = vin[u2 >> sb], x3 = vin[u3 >> sb];
+            vj[u0 & mask] = x0;
+            vj[u1 & mask] = x1;
+            vj[u2 & mask] = x2;
+            vj[u3 & mask] = x3;
+        }
+        for (; e < g1; e += kBlock) {
+            const uint32_t u = gl[e];
+            vj[u & mask] = vin[u >> sb];
+        }
+    } else {
+        const uint64_t *__restrict__ gl = a.gl64;
+        for (int64_t e = g0 + tid; e < g1; e += kBlock) {
+            const uint64_t u = gl[e];
+            vj[(uint32_t)u] = vin[(int64_t)(u >> 32)];
+        }
+    }
+}
+
+template <typename T, int L, bool KAHAN, int G>
 __device__ __forceinline__ void step_block(const StepArgs<T> &a, int64_t blk, int64_t t,
-                                           T *ic_lds, T *v_lds)
+                                           T *ic_lds, T *dyn)
 {
     constexpr int CPW = 64 / L;             // cells per wavefront per round
-    constexpr int CPR = kBlock / L;         // cells per workgroup per round
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int cw = lane / L, l = lane % L;
     const T *__restrict__ vin = a.vd[t & 1];
     T *__restrict__ vout = a.vd[(t & 1) ^ 1];
-    const int64_t cell0 = blk * kBlock;
+    int64_t cell0;
+    int ncell;
+    if constexpr (G == GATHER_STAGED) {
+        cell0 = a.blk_cell0[blk];
+        ncell = a.blk_ncell[blk];
+    } else {
+        cell0 = blk * kBlock;
+        ncell = kBlock;
+    }
 
-    if constexpr (VLDS) {
-        for (int64_t i = tid; i < a.n; i += kBlock) v_lds[i] = vin[i];
+    if constexpr (G == GATHER_VLDS) {
+        for (int64_t i = tid; i < a.n; i += kBlock) dyn[i] = vin[i];
+        __syncthreads();
+    } else if constexpr (G == GATHER_STAGED) {
+        stage_gather(a, blk, vin, dyn);
         __syncthreads();
     }
-    const T *__restrict__ vsrc = VLDS ? v_lds : vin;
 
-    // (1) gap-junction currents, L rounds of CPR cells
-#pragma unroll
-    for (int r = 0; r < L; r++) {
-        const int local = r * CPR + wave * CPW + cw;
+    // (1) gap-junction currents: slices of CPW cells, one per wavefront per round
+    const int nsl = ncell / CPW;
+    const int rounds = (nsl + 3) >> 2;
+    for (int r = 0; r < rounds; r++) {
+        const int sl = r * 4 + wave;                 // block-local slice (wave-uniform)
+        const int local = sl * CPW + cw;
         const int64_t cell = cell0 + local;
-        const int64_t slice = (cell0 + r * CPR + wave * CPW) / CPW;
         Acc<T, KAHAN> acc;
-        if (cell < a.n) {
+        if (sl < nsl && cell < a.n) {
             const int d = a.deg[cell];
             if (d > 0) {
-                const T vi = vsrc[cell];
-                const int64_t base = a.slice_ptr[slice] + cw * L + l;
+                const T vi = (G == GATHER_VLDS) ? dyn[cell] : vin[cell];
+                const int64_t base = a.slice_ptr[cell0 / CPW + sl] + cw * L + l;
                 const int nk = (d > l) ? (d - l + L - 1) / L : 0;
                 int kk = 0;
-                for (; kk + 2 <= nk; kk += 2) {
-                    const int64_t e0 = base + (int64_t)kk * 64, e1 = e0 + 64;
-                    const int j0 = a.col[e0], j1 = a.col[e1];
-                    const T w0 = a.w[e0], w1 = a.w[e1];
-                    const T v0 = vsrc[j0], v1 = vsrc[j1];
-                    acc.add(gj_term(vi, v0, w0));
-                    acc.add(gj_term(vi, v1, w1));
-                }
-                if (kk < nk) {
-                    const int64_t e0 = base + (int64_t)kk * 64;
-                    acc.add(gj_term(vi, vsrc[a.col[e0]], a.w[e0]));
+                if constexpr (G == GATHER_STAGED) {
+                    const T *__restrict__ vj = dyn + a.lrs[cell] + l;
+                    for (; kk + 2 <= nk; kk += 2) {
+                        const int64_t e0 = base + (int64_t)kk * 64;
+                        const T w0 = a.w[e0], w1 = a.w[e0 + 64];
+                        const T v0 = vj[kk * L], v1 = vj[(kk + 1) * L];
+                        acc.add(gj_term(vi, v0, w0));
+                        acc.add(gj_term(vi, v1, w1));
+                    }
+                    if (kk < nk) acc.add(gj_term(vi, vj[kk * L], a.w[base + (int64_t)kk * 64]));
+                } else {
+                    const T *__restrict__ vsrc = (G == GATHER_VLDS) ? dyn : vin;
+                    for (; kk + 2 <= nk; kk += 2) {
+                        const int64_t e0 = base + (int64_t)kk * 64, e1 = e0 + 64;
+                        const int j0 = a.col[e0], j1 = a.col[e1];
+                        const T w0 = a.w[e0], w1 = a.w[e1];
+                        const T v0 = vsrc[j0], v1 = vsrc[j1];
+                        acc.add(gj_term(vi, v0, w0));
+                        acc.add(gj_term(vi, v1, w1));
+                    }
+                    if (kk < nk) {
+                        const int64_t e0 = base + (int64_t)kk * 64;
+                        acc.add(gj_term(vi, vsrc[a.col[e0]], a.w[e0]));
+                    }
                 }
             }
         }
@@ -417,13 +493,13 @@ __device__ __forceinline__ void step_block(const StepArgs<T> &a, int64_t blk, in
                 acc.merge(os, oc);
             }
         }
-        if (l == 0) ic_lds[local] = acc.value();
+        if (l == 0 && sl < nsl) ic_lds[local] = acc.value();
     }
     __syncthreads();
 
     // (2) compartment update, one thread per cell
     const int64_t cell = cell0 + tid;
-    if (cell < a.n) {
+    if (tid < ncell && cell < a.n) {
         const int64_t np = a.npad;
         T *st = a.st;
         typename CellOf<T>::type c;
@@ -467,21 +543,21 @@ __device__ __forceinline__ void step_block(const StepArgs<T> &a, int64_t blk, in
         if (a.trace && (t % a.stride) == 0)
             a.trace[((t / a.stride) % a.trace_cap) * a.n + cell] = c.va;
     }
-    __syncthreads();   // ic_lds / v_lds are reused by the next block or step
+    __syncthreads();   // ic_lds / dyn are reused by the next block or step
 }
 
 /* ---------------------------------------------------------------------- */
 /* kernels                                                                  */
 /* ---------------------------------------------------------------------- */
 
-/* One step: step index = (*d_step if given) + k. */
-template <typename T, int L, bool KAHAN, bool VLDS>
+/* One step: step index = (*d_step if given) + k; one workgroup per block. */
+template <typename T, int L, bool KAHAN, int G>
 __global__ __launch_bounds__(kBlock) void k_step(StepArgs<T> a, int64_t k)
 {
     __shared__ T ic_lds[kBlock];
     extern __shared__ __align__(16) unsigned char dyn_lds[];
     const int64_t t = (a.d_step ? *a.d_step : 0) + k;
-    step_block<T, L, KAHAN, VLDS>(a, blockIdx.x, t, ic_lds, reinterpret_cast<T *>(dyn_lds));
+    step_block<T, L, KAHAN, G>(a, blockIdx.x, t, ic_lds, reinterpret_cast<T *>(dyn_lds));
 }
 
 /* Bounded grid barrier: every workgroup arrives once per barrier; waits at
@@ -523,21 +599,21 @@ __device__ __forceinline__ bool grid_barrier(GridBarrier *gb, unsigned target)
 }
 
 /* `nsteps` steps starting at *d_step; workgroups loop over cell blocks. */
-template <typename T, int L, bool KAHAN, bool VLDS>
+template <typename T, int L, bool KAHAN, int G>
 __global__ __launch_bounds__(kBlock) void k_persist(StepArgs<T> a, GridBarrier *gb, int nsteps)
 {
     __shared__ T ic_lds[kBlock];
     extern __shared__ __align__(16) unsigned char dyn_lds[];
     const int64_t t0 = *a.d_step;
-    const int64_t nblk = a.npad / kBlock;
     for (int k = 0; k < nsteps; k++) {
-        for (int64_t b = blockIdx.x; b < nblk; b += gridDim.x)
-            step_block<T, L, KAHAN, VLDS>(a, b, t0 + k, ic_lds, reinterpret_cast<T *>(dyn_lds));
+        for (int64_t b = blockIdx.x; b < a.nblocks; b += gridDim.x)
+            step_block<T, L, KAHAN, G>(a, b, t0 + k, ic_lds, reinterpret_cast<T *>(dyn_lds));
         if (k + 1 < nsteps && !grid_barrier(gb, (unsigned)(k + 1) * gridDim.x)) return;
     }
 }
 
-/* I_c of every cell from V_dend at step *d_step (no state change). */
+/* I_c of every cell from V_dend at step t (no state change); direct gathers
+ * over the uniform 256-cell blocks of the sliced-ELL layout. */
 template <typename T, int L, bool KAHAN>
 __global__ __launch_bounds__(kBlock) void k_gap_currents(StepArgs<T> a, int64_t t, T *out)
 {

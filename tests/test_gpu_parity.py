@@ -98,16 +98,25 @@ def test_random_network_vs_oracle(prec, tol):
 MODES = [(I.EXEC_STEP, "step"), (I.EXEC_GRAPH, "graph"), (I.EXEC_PERSISTENT, "persistent")]
 
 
+GATHERS = [(I.GATHER_DIRECT, 76, "direct"), (I.GATHER_LDS, 76, "lds"),
+           (I.GATHER_STAGED, 8, "staged8k"), (I.GATHER_STAGED, 76, "staged76k"),
+           (I.GATHER_STAGED, 150, "staged150k")]
+
+
 @pytest.mark.parametrize("mode,mname", MODES)
 @pytest.mark.parametrize("L", [1, 2, 4, 8, 16])
-@pytest.mark.parametrize("vlds", [0, 1])
+@pytest.mark.parametrize("gather,kb,gname", GATHERS)
 @pytest.mark.parametrize("prec", ["fp64", "fp32k"])
-def test_exec_strategies_agree(mode, mname, L, vlds, prec):
-    say(f"exec {mname} L={L} vlds={vlds} {prec}")
+def test_exec_strategies_agree(mode, mname, L, gather, kb, gname, prec):
+    say(f"exec {mname} L={L} {gname} {prec}")
     n, steps = 600, 230
     rp, col, w, gcal, iapp = _random_case(n, p=0.15, seed=9)
+    if gather == I.GATHER_STAGED and kb == 8:
+        # 8 KiB holds 2048 FP32 / 1024 FP64 entries: a sparser graph so a
+        # 64-cell group fits, giving many small blocks
+        rp, col, w, gcal, iapp = _random_case(n, p=0.02, seed=9)
     net = make_net(n, prec, rp, col, w, gcal, iapp)
-    net.set_exec(mode=mode, lanes_per_cell=L, v_in_lds=vlds, steps_per_launch=50)
+    net.set_exec(mode=mode, lanes_per_cell=L, gather=gather, lds_kb=kb, steps_per_launch=50)
     out = net.advance(steps, trace_stride=10)
     vd = net.get_state("V_dend")
     net.close()
@@ -304,6 +313,7 @@ def test_sparse_100k_full_size_precision():
     t32 = nets["fp32k"].advance(1999, trace_stride=1)
     err = np.abs(t64 - t32).max()
     say(f"  max |dV| FP32+compensated vs FP64 over 100 ms: {err:.3e} mV")
+    say(f"  first-step max |dV| vs CPU FP64 oracle: {np.abs(one64 - ref).max():.3e} mV")
     for net in nets.values():
         net.close()
     assert err <= 1e-3
@@ -346,6 +356,10 @@ def test_driver_reference_format():
                            capture_output=True, text=True, timeout=120)
         assert r.returncode == 0, r.stdout
         lines = open(os.path.join(d, "InferiorOlive_Output.txt")).read().splitlines()
+    # rewriting line 1 with the time (OMP:723-732) over the padded
+    # placeholder leaves the placeholder's tail as a blank line, as the
+    # reference does
+    lines = [ln for ln in lines if ln.strip()]
     assert lines[0].startswith("Execution Time for Simulation in ms: ")
     assert lines[1] == "#simSteps Input(Iapp) Output(V_axon)0 1 2 3 4 "
     assert len(lines) == 2 + 4            # 400 steps, a record every 100
