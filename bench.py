@@ -82,8 +82,9 @@ def main():
     ap.add_argument("--stride", type=int, default=100)
     ap.add_argument("--mode", default="auto", choices=["auto", "graph", "persistent", "step", "tune"])
     ap.add_argument("--lanes", type=int, default=0)
-    ap.add_argument("--gather", default="auto", choices=["auto", "direct", "lds", "staged"])
+    ap.add_argument("--gather", default="auto", choices=["auto", "direct", "lds", "staged", "tiled"])
     ap.add_argument("--lds-kb", type=int, default=0)
+    ap.add_argument("--tile-cells", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     args = ap.parse_args()
@@ -104,10 +105,12 @@ def main():
         W.load_into(net, wl)
         if args.lanes:
             net.set_exec(lanes_per_cell=args.lanes)
-        if args.gather != "auto":
-            net.set_exec(gather={"direct": 0, "lds": 1, "staged": 2}[args.gather])
         if args.lds_kb:
             net.set_exec(lds_kb=args.lds_kb)
+        if args.tile_cells:
+            net.set_exec(tile_cells=args.tile_cells)
+        if args.gather != "auto":
+            net.set_exec(gather={"direct": 0, "lds": 1, "staged": 2, "tiled": 3}[args.gather])
         if args.mode == "graph":
             net.set_exec(mode=I.EXEC_GRAPH)
         elif args.mode == "persistent":

@@ -186,6 +186,9 @@ int infoli_gap_currents(infoli_t *ctx, double *i_c);
 #define INFOLI_GATHER_LDS       1  /* whole V_dend array staged in LDS (small n) */
 #define INFOLI_GATHER_STAGED    2  /* per cell block: senders sorted, copied to
                                       LDS, then read in each row's order        */
+#define INFOLI_GATHER_TILED     3  /* receiver block x sender chunk tiles: the
+                                      chunk's V_dend in LDS, per-tile partial
+                                      sums merged by the compartment kernel     */
 
 typedef struct infoli_exec {
     int     mode;            /* INFOLI_EXEC_*                                 */
@@ -193,7 +196,9 @@ typedef struct infoli_exec {
     int     block;           /* threads per workgroup (256)                   */
     int     steps_per_launch;/* graph length / persistent steps per launch    */
     int     gather;          /* INFOLI_GATHER_*                               */
-    int     lds_kb;          /* staged gather: LDS budget per workgroup, KiB  */
+    int     lds_kb;          /* staged / tiled: LDS budget per workgroup, KiB */
+    int     tile_cells;      /* tiled: receivers per tile (multiple of 64)    */
+    int     ode_split;       /* tiled: one wavefront per compartment (0/1)    */
 } infoli_exec;
 
 int infoli_get_exec(const infoli_t *ctx, infoli_exec *out);

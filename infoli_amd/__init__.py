@@ -17,6 +17,7 @@ from .network import (  # noqa: F401
     GATHER_DIRECT,
     GATHER_LDS,
     GATHER_STAGED,
+    GATHER_TILED,
     VARS,
     InfoliError,
     Network,

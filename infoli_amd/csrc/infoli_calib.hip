@@ -1,0 +1,156 @@
+/*
+ * infoli_calib.hip -- measurement helpers for the benchmark write-up.
+ *
+ *   infoli_calib.x copy   [MiB]      plain device copy of a known size: the
+ *                                    reference point for scaling rocprofv3's
+ *                                    FETCH_SIZE / WRITE_SIZE counters
+ *   infoli_calib.x gather [table]    random 4-byte gathers from a table of
+ *                                    `table` floats (global memory) and, when
+ *                                    it fits, from LDS
+ *   infoli_calib.x exp               throughput of the hardware exp2 (v_exp_f32)
+ *
+ * Every index is drawn in range on the host; grids are sized to the buffers.
+ */
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#define CHK(x)                                                                    \
+    do {                                                                          \
+        hipError_t e_ = (x);                                                      \
+        if (e_ != hipSuccess) {                                                   \
+            fprintf(stderr, "%s: %s\n", #x, hipGetErrorString(e_));               \
+            exit(1);                                                              \
+        }                                                                         \
+    } while (0)
+
+__global__ void copy_k(const float4 *__restrict__ a, float4 *__restrict__ b, long n)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long s = (long)gridDim.x * blockDim.x;
+    for (; i < n; i += s) b[i] = a[i];
+}
+
+__global__ void gather_global(const int *__restrict__ idx, const float *__restrict__ v, float *out,
+                              int per, long nthreads)
+{
+    const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    float acc = 0;
+    for (int k = 0; k < per; k++) acc += v[idx[tid + (long)k * nthreads]];
+    out[tid] = acc;
+}
+
+__global__ void gather_lds(const int *__restrict__ idx, const float *__restrict__ v, int nv, float *out,
+                           int per, long nthreads)
+{
+    extern __shared__ float sv[];
+    for (int i = threadIdx.x; i < nv; i += blockDim.x) sv[i] = v[i];
+    __syncthreads();
+    const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    float acc = 0;
+    for (int k = 0; k < per; k++) acc += sv[idx[tid + (long)k * nthreads]];
+    out[tid] = acc;
+}
+
+__global__ void exp_rate(float *out, int iters)
+{
+    float a = threadIdx.x * 1e-3f, b = a + 0.5f, c = a + 0.25f, d = a + 0.125f;
+    for (int i = 0; i < iters; i++) {
+        a = __builtin_amdgcn_exp2f(a) * -0.5f;
+        b = __builtin_amdgcn_exp2f(b) * -0.5f;
+        c = __builtin_amdgcn_exp2f(c) * -0.5f;
+        d = __builtin_amdgcn_exp2f(d) * -0.5f;
+    }
+    out[(long)blockIdx.x * blockDim.x + threadIdx.x] = a + b + c + d;
+}
+
+static float time_ms(hipEvent_t e0, hipEvent_t e1)
+{
+    float ms = 0;
+    CHK(hipEventSynchronize(e1));
+    CHK(hipEventElapsedTime(&ms, e0, e1));
+    return ms;
+}
+
+int main(int argc, char **argv)
+{
+    const char *what = argc > 1 ? argv[1] : "copy";
+    hipEvent_t e0, e1;
+    CHK(hipEventCreate(&e0));
+    CHK(hipEventCreate(&e1));
+    if (!strcmp(what, "copy")) {
+        const long mib = argc > 2 ? atol(argv[2]) : 1024;
+        const long n4 = mib * (1L << 20) / 16;
+        float4 *a, *b;
+        CHK(hipMalloc(&a, n4 * 16));
+        CHK(hipMalloc(&b, n4 * 16));
+        CHK(hipMemset(a, 0, n4 * 16));
+        for (int rep = 0; rep < 3; rep++) {
+            CHK(hipEventRecord(e0));
+            hipLaunchKernelGGL(copy_k, dim3(256 * 16), dim3(256), 0, 0, a, b, n4);
+            CHK(hipGetLastError());
+            CHK(hipEventRecord(e1));
+            const float ms = time_ms(e0, e1);
+            printf("copy %ld MiB: %.3f ms, %.1f GB/s read+write\n", mib, ms, 2.0 * n4 * 16 / ms / 1e6);
+        }
+        CHK(hipFree(a));
+        CHK(hipFree(b));
+    } else if (!strcmp(what, "gather")) {
+        const long nv = argc > 2 ? atol(argv[2]) : 100000;
+        const long nthreads = 1L << 20;
+        const int per = 16;
+        const long ne = nthreads * per;
+        std::vector<int> hidx(ne);
+        std::mt19937 rng(1);
+        std::uniform_int_distribution<int> U(0, (int)nv - 1);
+        for (long i = 0; i < ne; i++) hidx[i] = U(rng);
+        std::vector<float> hv(nv, 1.0f);
+        int *idx;
+        float *v, *out;
+        CHK(hipMalloc(&idx, ne * 4));
+        CHK(hipMalloc(&v, nv * 4));
+        CHK(hipMalloc(&out, nthreads * 4));
+        CHK(hipMemcpy(idx, hidx.data(), ne * 4, hipMemcpyHostToDevice));
+        CHK(hipMemcpy(v, hv.data(), nv * 4, hipMemcpyHostToDevice));
+        for (int rep = 0; rep < 2; rep++) {
+            CHK(hipEventRecord(e0));
+            hipLaunchKernelGGL(gather_global, dim3(nthreads / 256), dim3(256), 0, 0, idx, v, out, per, nthreads);
+            CHK(hipGetLastError());
+            CHK(hipEventRecord(e1));
+            const float ms = time_ms(e0, e1);
+            if (rep) printf("gather global, table %ld floats: %.3f ms, %.1f G gathers/s\n", nv, ms, ne / ms / 1e6);
+        }
+        if (nv * 4 <= 64 * 1024) {
+            for (int rep = 0; rep < 2; rep++) {
+                CHK(hipEventRecord(e0));
+                hipLaunchKernelGGL(gather_lds, dim3(nthreads / 256), dim3(256), nv * 4, 0, idx, v, (int)nv, out,
+                                   per, nthreads);
+                CHK(hipGetLastError());
+                CHK(hipEventRecord(e1));
+                const float ms = time_ms(e0, e1);
+                if (rep) printf("gather LDS, table %ld floats: %.3f ms, %.1f G gathers/s\n", nv, ms, ne / ms / 1e6);
+            }
+        }
+    } else if (!strcmp(what, "exp")) {
+        const long nthreads = 1L << 20;
+        const int iters = 4096;
+        float *out;
+        CHK(hipMalloc(&out, nthreads * 4));
+        for (int rep = 0; rep < 2; rep++) {
+            CHK(hipEventRecord(e0));
+            hipLaunchKernelGGL(exp_rate, dim3(nthreads / 256), dim3(256), 0, 0, out, iters);
+            CHK(hipGetLastError());
+            CHK(hipEventRecord(e1));
+            const float ms = time_ms(e0, e1);
+            if (rep) printf("exp2 (+1 mul each): %.1f G exp/s\n", (double)nthreads * iters * 4 / ms / 1e6);
+        }
+    } else {
+        fprintf(stderr, "usage: infoli_calib.x copy [MiB] | gather [table] | exp\n");
+        return 2;
+    }
+    return 0;
+}

@@ -61,6 +61,8 @@ struct infoli_ctx {
 
     void *d_state = nullptr;
     void *d_vd[2] = {nullptr, nullptr};
+    void *d_vs[2] = {nullptr, nullptr};
+    void *d_va[2] = {nullptr, nullptr};
     int32_t *d_deg = nullptr;
     int64_t *d_slice_ptr = nullptr;
     int32_t *d_col = nullptr;
@@ -77,6 +79,16 @@ struct infoli_ctx {
     int64_t *d_blk_gbase = nullptr;
     uint32_t *d_gl32 = nullptr;
     uint64_t *d_gl64 = nullptr;
+
+    // 2D-tiled layout (built for one lanes-per-cell, chunk size and block size)
+    bool tiled_valid = false;
+    int tiled_L = 0, tiled_kb = 0, tiled_R = 0;
+    int64_t tl_ntiles = 0, tl_nrb = 0, tl_C = 0;
+    TileDesc *d_tiles = nullptr;
+    int32_t *d_rb_tile0 = nullptr;
+    int64_t *d_tsp = nullptr;
+    uint16_t *d_rlist = nullptr, *d_rdeg = nullptr, *d_tcol = nullptr;
+    void *d_tw = nullptr, *d_part = nullptr;
 
     void *d_iapp_base = nullptr;
     uint8_t *d_mask = nullptr;
@@ -167,6 +179,10 @@ StepArgs<T> make_args(const infoli_ctx *c, int64_t stride)
     a.st = (T *)c->d_state;
     a.vd[0] = (T *)c->d_vd[0];
     a.vd[1] = (T *)c->d_vd[1];
+    a.vs[0] = (T *)c->d_vs[0];
+    a.vs[1] = (T *)c->d_vs[1];
+    a.va[0] = (T *)c->d_va[0];
+    a.va[1] = (T *)c->d_va[1];
     a.deg = c->d_deg;
     a.slice_ptr = c->d_slice_ptr;
     a.col = c->d_col;
@@ -194,6 +210,18 @@ StepArgs<T> make_args(const infoli_ctx *c, int64_t stride)
         a.sb = c->st_sb;
     } else {
         a.nblocks = c->npad / kBlock;
+    }
+    if (c->exec.gather == GATHER_TILED) {
+        a.tiles = c->d_tiles;
+        a.rb_tile0 = c->d_rb_tile0;
+        a.tsp = c->d_tsp;
+        a.rlist = c->d_rlist;
+        a.rdeg = c->d_rdeg;
+        a.tcol = c->d_tcol;
+        a.tw = (const T *)c->d_tw;
+        a.part = c->d_part;
+        a.rcells = c->tiled_R;
+        a.ccells = (int32_t)c->tl_C;
     }
     a.m = make_model<T>(c->params);
     return a;
@@ -237,6 +265,24 @@ const void *const k_persist_tab[3][3][5] = {
     INFOLI_G_TABLE(k_persist, float, false),
     INFOLI_G_TABLE(k_persist, float, true),
 };
+/* [precision][threads: 256, 512, 1024][L] */
+#define INFOLI_NT_TABLE(T, K) \
+    {INFOLI_L_TABLE(k_tile_gj, T, K, 256), INFOLI_L_TABLE(k_tile_gj, T, K, 512), \
+     INFOLI_L_TABLE(k_tile_gj, T, K, 1024)}
+const void *const k_tile_gj_tab[3][3][5] = {
+    INFOLI_NT_TABLE(double, false),
+    INFOLI_NT_TABLE(float, false),
+    INFOLI_NT_TABLE(float, true),
+};
+int ntidx(int nt) { return nt == 1024 ? 2 : nt == 512 ? 1 : 0; }
+const void *const k_tile_ode_tab[3] = {
+    (const void *)k_tile_ode<double, false>, (const void *)k_tile_ode<float, false>,
+    (const void *)k_tile_ode<float, true>,
+};
+const void *const k_tile_ode3_tab[3] = {
+    (const void *)k_tile_ode3<double, false>, (const void *)k_tile_ode3<float, false>,
+    (const void *)k_tile_ode3<float, true>,
+};
 const void *const k_gap_tab[3][5] = {
     INFOLI_L_TABLE(k_gap_currents, double, false),
     INFOLI_L_TABLE(k_gap_currents, float, false),
@@ -249,7 +295,8 @@ size_t dyn_lds_bytes(const infoli_ctx *c)
 {
     switch (c->exec.gather) {
     case GATHER_VLDS: return (size_t)c->n * c->tsize;
-    case GATHER_STAGED: return (size_t)std::max<int64_t>(c->st_elds, 1) * c->tsize;
+    case GATHER_STAGED:   // + slack: padded slots of a row's last slice read past its end
+        return (size_t)(std::max<int64_t>(c->st_elds, 1) + c->max_deg + 64) * c->tsize;
     default: return 0;
     }
 }
@@ -268,6 +315,21 @@ int64_t n_blocks(const infoli_ctx *c)
 
 int launch_step(infoli_ctx *c, const void *args, int64_t k)
 {
+    if (c->exec.gather == GATHER_TILED) {
+        const void *fg = k_tile_gj_tab[c->prec][ntidx(c->exec.block)][lidx(c->exec.lanes_per_cell)];
+        const void *fo = k_tile_ode_tab[c->prec];
+        int64_t kk = k;
+        void *kargs[] = {const_cast<void *>(args), &kk};
+        const size_t lds = (size_t)c->tl_C * c->tsize;
+        allow_dyn_lds(fg, lds);
+        HIPCHK(hipLaunchKernel(fg, dim3((unsigned)c->tl_ntiles), dim3(c->exec.block), kargs, lds, c->stream));
+        if (c->exec.ode_split)
+            HIPCHK(hipLaunchKernel(k_tile_ode3_tab[c->prec], dim3((unsigned)(c->npad / 64)), dim3(192), kargs, 0,
+                                   c->stream));
+        else
+            HIPCHK(hipLaunchKernel(fo, dim3((unsigned)(c->npad / kBlock)), dim3(kBlock), kargs, 0, c->stream));
+        return INFOLI_OK;
+    }
     const void *fn = k_step_tab[c->prec][c->exec.gather][lidx(c->exec.lanes_per_cell)];
     const int64_t nblk = n_blocks(c);
     int64_t kk = k;
@@ -373,6 +435,141 @@ void free_staged(infoli_ctx *c)
  * the LDS budget; per block the list of (sender, LDS slot) sorted by
  * sender, slot = the entry's position in the block's compact row-major
  * list; and per cell its LDS row start. */
+void free_tiled(infoli_ctx *c)
+{
+    void *bufs[] = {c->d_tiles, c->d_rb_tile0, c->d_tsp, c->d_rlist, c->d_rdeg, c->d_tcol, c->d_tw, c->d_part};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    c->d_tiles = nullptr;
+    c->d_rb_tile0 = nullptr;
+    c->d_tsp = nullptr;
+    c->d_rlist = c->d_rdeg = c->d_tcol = nullptr;
+    c->d_tw = c->d_part = nullptr;
+    c->tiled_valid = false;
+    c->tl_ntiles = c->tl_nrb = c->tl_C = 0;
+}
+
+/* chunk of senders per tile for an LDS budget: a multiple of 64 cells */
+int64_t tiled_chunk(const infoli_ctx *c, int kb)
+{
+    const int64_t cap = (int64_t)kb * 1024 / (int64_t)c->tsize / 64 * 64;
+    const int64_t nch = (c->n + cap - 1) / cap;
+    return std::min<int64_t>(cap, round_up((c->n + nch - 1) / nch, 64));
+}
+
+/* 2D-tiled layout: receiver blocks of R cells x sender chunks of C cells.
+ * A tile exists for every (block, chunk) pair with a junction.  In a tile
+ * the block's R receivers are listed by in-chunk degree, descending, and
+ * their in-chunk junctions (in row order) are stored sliced-ELL with
+ * 16-bit chunk-local sender ids. */
+template <typename T>
+int upload_tiled(infoli_ctx *c, int L, int R, int kb)
+{
+    const int CPW = 64 / L;
+    const int64_t n = c->n;
+    const int64_t C = tiled_chunk(c, kb);
+    const int64_t nrb = (n + R - 1) / R;
+    std::vector<TileDesc> tiles;
+    std::vector<int32_t> rbt0(nrb + 1, 0);
+    std::vector<int64_t> tsp(1, 0);
+    std::vector<uint16_t> rlist, rdeg, tcol;
+    std::vector<T> tw;
+    for (int64_t rb = 0; rb < nrb; rb++) {
+        rbt0[rb] = (int32_t)tiles.size();
+        const int64_t c0 = rb * R, c1 = std::min<int64_t>(n, c0 + R);
+        // per sender chunk: the block's junctions into it, receivers ascending, row order
+        const int64_t nchunks = (n + C - 1) / C;
+        std::vector<std::vector<int64_t>> ent(nchunks);
+        std::vector<std::vector<int32_t>> entr(nchunks);
+        for (int64_t i = c0; i < c1; i++)
+            for (int64_t e = c->row_ptr[i]; e < c->row_ptr[i + 1]; e++) {
+                ent[c->col[e] / C].push_back(e);
+                entr[c->col[e] / C].push_back((int32_t)(i - c0));
+            }
+        for (int64_t ch = 0; ch < nchunks; ch++) {
+            if (ent[ch].empty()) continue;
+            std::vector<int> deg(R, 0);
+            std::vector<int64_t> first(R, -1);
+            for (size_t q = 0; q < ent[ch].size(); q++) {
+                const int r = entr[ch][q];
+                if (first[r] < 0) first[r] = (int64_t)q;
+                deg[r]++;
+            }
+            std::vector<int> rs(R);
+            for (int r = 0; r < R; r++) rs[r] = r;
+            std::stable_sort(rs.begin(), rs.end(), [&](int x, int y) { return deg[x] > deg[y]; });
+            TileDesc td;
+            td.rb = (int32_t)rb;
+            td.ch = (int32_t)ch;
+            td.nsl = R / CPW;
+            td.pad = 0;
+            td.sl0 = (int64_t)tsp.size() - 1;
+            td.rl0 = (int64_t)rlist.size();
+            for (int r : rs) {
+                rlist.push_back((uint16_t)r);
+                if (deg[r] > 65535) return infoli_fail(INFOLI_ERR_ARG, "tiled: in-chunk degree above 65535");
+                rdeg.push_back((uint16_t)deg[r]);
+            }
+            for (int sl = 0; sl < td.nsl; sl++) {
+                int md = 0;
+                for (int cw = 0; cw < CPW; cw++) md = std::max(md, deg[rs[sl * CPW + cw]]);
+                const int64_t base = tsp.back();
+                const int64_t width = (md + L - 1) / L;
+                tsp.push_back(base + width * 64);
+                tcol.resize(tsp.back(), 0);
+                tw.resize(tsp.back(), T(0));
+                for (int cw = 0; cw < CPW; cw++) {
+                    const int r = rs[sl * CPW + cw];
+                    if (deg[r] == 0) continue;
+                    const int64_t f = first[r];
+                    for (int p = 0; p < deg[r]; p++) {
+                        const int64_t e = ent[ch][f + p];
+                        const int64_t slot = base + (int64_t)(p / L) * 64 + cw * L + (p % L);
+                        tcol[slot] = (uint16_t)(c->col[e] - ch * C);
+                        tw[slot] = (T)c->w[e];
+                    }
+                }
+            }
+            tiles.push_back(td);
+        }
+    }
+    rbt0[nrb] = (int32_t)tiles.size();
+    if (tiles.empty()) {   // no junctions at all: one empty tile keeps the kernels well-formed
+        TileDesc td{0, 0, 0, 0, 0, 0};
+        tiles.push_back(td);
+        std::fill(rbt0.begin(), rbt0.end(), 0);
+    }
+    free_tiled(c);
+    const size_t psz = is_f32(c) && c->prec == INFOLI_FP32_KAHAN ? 8 : c->tsize;
+    HIPCHK(hipMalloc(&c->d_tiles, sizeof(TileDesc) * tiles.size()));
+    HIPCHK(hipMalloc(&c->d_rb_tile0, sizeof(int32_t) * rbt0.size()));
+    HIPCHK(hipMalloc(&c->d_tsp, sizeof(int64_t) * tsp.size()));
+    HIPCHK(hipMalloc(&c->d_rlist, sizeof(uint16_t) * std::max<size_t>(rlist.size(), 1)));
+    HIPCHK(hipMalloc(&c->d_rdeg, sizeof(uint16_t) * std::max<size_t>(rdeg.size(), 1)));
+    HIPCHK(hipMalloc(&c->d_tcol, sizeof(uint16_t) * std::max<size_t>(tcol.size(), 1)));
+    HIPCHK(hipMalloc(&c->d_tw, sizeof(T) * std::max<size_t>(tw.size(), 1)));
+    HIPCHK(hipMalloc(&c->d_part, psz * tiles.size() * (size_t)R));
+    HIPCHK(hipMemcpy(c->d_tiles, tiles.data(), sizeof(TileDesc) * tiles.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_rb_tile0, rbt0.data(), sizeof(int32_t) * rbt0.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_tsp, tsp.data(), sizeof(int64_t) * tsp.size(), hipMemcpyHostToDevice));
+    if (!rlist.empty()) {
+        HIPCHK(hipMemcpy(c->d_rlist, rlist.data(), sizeof(uint16_t) * rlist.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->d_rdeg, rdeg.data(), sizeof(uint16_t) * rdeg.size(), hipMemcpyHostToDevice));
+    }
+    if (!tcol.empty()) {
+        HIPCHK(hipMemcpy(c->d_tcol, tcol.data(), sizeof(uint16_t) * tcol.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->d_tw, tw.data(), sizeof(T) * tw.size(), hipMemcpyHostToDevice));
+    }
+    c->tl_ntiles = (int64_t)tiles.size();
+    c->tl_nrb = nrb;
+    c->tl_C = C;
+    c->tiled_L = L;
+    c->tiled_kb = kb;
+    c->tiled_R = R;
+    c->tiled_valid = true;
+    return INFOLI_OK;
+}
+
 int upload_staged(infoli_ctx *c, int kb)
 {
     const int64_t n = c->n, npad = c->npad;
@@ -463,6 +660,13 @@ int ensure_layout(infoli_ctx *c)
         HIPCHK(hipStreamSynchronize(c->stream));
         RCHK(upload_staged(c, c->exec.lds_kb));
     }
+    if (c->exec.gather == GATHER_TILED &&
+        (!c->tiled_valid || c->tiled_kb != c->exec.lds_kb || c->tiled_L != L || c->tiled_R != c->exec.tile_cells)) {
+        HIPCHK(hipSetDevice(c->device));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        RCHK(is_f32(c) ? upload_tiled<float>(c, L, c->exec.tile_cells, c->exec.lds_kb)
+                       : upload_tiled<double>(c, L, c->exec.tile_cells, c->exec.lds_kb));
+    }
     return INFOLI_OK;
 }
 
@@ -495,6 +699,30 @@ bool max_block_entries_fit(const infoli_ctx *c, int kb)
     return true;
 }
 
+/* mean number of sender chunks (tiles) per receiver block of the tiled layout */
+double tiles_per_block(const infoli_ctx *c, int R, int64_t C)
+{
+    const int64_t nrb = (c->n + R - 1) / R, nch = (c->n + C - 1) / C;
+    std::vector<int64_t> seen(nch, -1);
+    int64_t tiles = 0;
+    for (int64_t rb = 0; rb < nrb; rb++) {
+        const int64_t c1 = std::min<int64_t>(c->n, (rb + 1) * R);
+        for (int64_t e = c->row_ptr[rb * R]; e < c->row_ptr[c1]; e++) {
+            const int64_t ch = c->col[e] / C;
+            if (seen[ch] != rb) {
+                seen[ch] = rb;
+                tiles++;
+            }
+        }
+    }
+    return nrb > 0 ? (double)tiles / (double)nrb : 0.0;
+}
+
+/* Heuristic launch strategy (infoli_autotune measures instead):
+ *  - tiled gather when the per-tile partial sums (one pair per receiver per
+ *    tile) cost at most half the junction stream they replace;
+ *  - else the staged gather when a 64-cell group fits the LDS budget;
+ *  - else direct gathers. */
 void default_exec(infoli_ctx *c)
 {
     const double mean_deg = c->n > 0 ? (double)c->col.size() / (double)c->n : 0;
@@ -503,12 +731,24 @@ void default_exec(infoli_ctx *c)
     c->exec.block = kBlock;
     c->exec.steps_per_launch = 100;
     c->exec.lds_kb = kDefaultLdsKb;
-    if ((size_t)c->n * c->tsize <= kVldsMaxBytes && mean_deg >= 64)
-        c->exec.gather = GATHER_VLDS;
-    else if (mean_deg >= 4 && max_block_entries_fit(c, c->exec.lds_kb))
+    c->exec.tile_cells = 2048;
+    c->exec.ode_split = 1;
+    c->exec.gather = GATHER_DIRECT;
+    if (mean_deg < 4) return;
+    const int tkb = 48;
+    const int64_t C = tiled_chunk(c, tkb);
+    const double psz = c->prec == INFOLI_FP32 ? 4.0 : 8.0;
+    if (tiles_per_block(c, c->exec.tile_cells, C) * psz <= 0.5 * mean_deg * (2.0 + (double)c->tsize)) {
+        c->exec.gather = GATHER_TILED;
+        c->exec.lds_kb = tkb;
+        c->exec.block = 1024;
+        c->exec.lanes_per_cell = 1;
+        return;
+    }
+    if (max_block_entries_fit(c, 40)) {
         c->exec.gather = GATHER_STAGED;
-    else
-        c->exec.gather = GATHER_DIRECT;
+        c->exec.lds_kb = 40;
+    }
 }
 
 int ensure_stage(infoli_ctx *c, size_t bytes)
@@ -611,6 +851,8 @@ extern "C" int infoli_create(infoli_t **out, const infoli_params *params, int64_
     const size_t plane = (size_t)c->npad * c->tsize;
     if (hipMalloc(&c->d_state, plane * kNV) != hipSuccess ||
         hipMalloc(&c->d_vd[0], plane) != hipSuccess || hipMalloc(&c->d_vd[1], plane) != hipSuccess ||
+        hipMalloc(&c->d_vs[0], plane) != hipSuccess || hipMalloc(&c->d_vs[1], plane) != hipSuccess ||
+        hipMalloc(&c->d_va[0], plane) != hipSuccess || hipMalloc(&c->d_va[1], plane) != hipSuccess ||
         hipMalloc(&c->d_deg, sizeof(int32_t) * c->npad) != hipSuccess ||
         hipMalloc(&c->d_step, sizeof(int64_t)) != hipSuccess ||
         hipMalloc(&c->d_bar, sizeof(GridBarrier)) != hipSuccess ||
@@ -618,7 +860,9 @@ extern "C" int infoli_create(infoli_t **out, const infoli_params *params, int64_
         return fail(infoli_fail(INFOLI_ERR_NOMEM, "infoli_create: device allocation of %lld cells failed",
                                 (long long)n_cells));
     if (hipMemset(c->d_state, 0, plane * kNV) != hipSuccess ||
-        hipMemset(c->d_vd[0], 0, plane) != hipSuccess || hipMemset(c->d_vd[1], 0, plane) != hipSuccess)
+        hipMemset(c->d_vd[0], 0, plane) != hipSuccess || hipMemset(c->d_vd[1], 0, plane) != hipSuccess ||
+        hipMemset(c->d_vs[0], 0, plane) != hipSuccess || hipMemset(c->d_vs[1], 0, plane) != hipSuccess ||
+        hipMemset(c->d_va[0], 0, plane) != hipSuccess || hipMemset(c->d_va[1], 0, plane) != hipSuccess)
         return fail(infoli_fail(INFOLI_ERR_HIP, "infoli_create: memset failed"));
 
     // initial state (OMP:267-292): float readings for FP32, double for FP64
@@ -627,7 +871,8 @@ extern "C" int infoli_create(infoli_t **out, const infoli_params *params, int64_
         const double x = precision == INFOLI_FP64 ? params->v[INFOLI_P_STATE0 + v]
                                                   : (double)params->f[INFOLI_P_STATE0 + v];
         std::fill(vals.begin(), vals.end(), x);
-        void *dst = (v == V_DEND) ? c->d_vd[0] : (char *)c->d_state + (size_t)v * plane;
+        void *dst = (v == V_DEND) ? c->d_vd[0] : (v == V_SOMA) ? c->d_vs[0] : (v == V_AXON) ? c->d_va[0]
+                                                                         : (char *)c->d_state + (size_t)v * plane;
         rc = upload_values(c, dst, vals.data(), n_cells);
         if (rc) return fail(rc);
     }
@@ -647,11 +892,12 @@ extern "C" void infoli_destroy(infoli_t *c)
     for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
-    void *bufs[] = {c->d_state, c->d_vd[0], c->d_vd[1], c->d_deg, c->d_slice_ptr, c->d_col, c->d_w,
+    void *bufs[] = {c->d_state, c->d_vd[0], c->d_vd[1], c->d_vs[0], c->d_vs[1], c->d_va[0], c->d_va[1], c->d_deg, c->d_slice_ptr, c->d_col, c->d_w,
                     c->d_iapp_base, c->d_mask, c->d_tab, c->d_trace, c->d_step, c->d_bar, c->d_flag};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     free_staged(c);
+    free_tiled(c);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -700,6 +946,7 @@ extern "C" int infoli_set_connectivity_csr(infoli_t *c, const int64_t *row_ptr,
     c->w.assign(w ? w : nullptr, w ? w + nnz : nullptr);
     c->max_deg = max_deg;
     c->staged_valid = false;
+    c->tiled_valid = false;
     const int keep_mode = c->exec.mode;
     default_exec(c);
     if (keep_mode != INFOLI_EXEC_GRAPH) c->exec.mode = keep_mode;
@@ -796,6 +1043,8 @@ extern "C" int infoli_set_iapp_table(infoli_t *c, int64_t t_first, int64_t n_ste
 static void *var_ptr(infoli_ctx *c, int var)
 {
     if (var == V_DEND) return c->d_vd[c->step & 1];
+    if (var == V_SOMA) return c->d_vs[c->step & 1];
+    if (var == V_AXON) return c->d_va[c->step & 1];
     return (char *)c->d_state + (size_t)var * c->npad * c->tsize;
 }
 
@@ -845,18 +1094,30 @@ extern "C" int infoli_set_exec(infoli_t *c, const infoli_exec *e)
         return infoli_fail(INFOLI_ERR_ARG, "infoli_set_exec: mode %d not available", x.mode);
     if (lidx(x.lanes_per_cell) < 0)
         return infoli_fail(INFOLI_ERR_ARG, "lanes_per_cell %d not in {1,2,4,8,16}", x.lanes_per_cell);
-    if (x.block != kBlock) return infoli_fail(INFOLI_ERR_ARG, "block must be %d", kBlock);
+    if (x.gather == GATHER_TILED ? (x.block != 256 && x.block != 512 && x.block != 1024) : x.block != kBlock)
+        return infoli_fail(INFOLI_ERR_ARG, "block %d: must be %d (256, 512 or 1024 for the tiled gather)",
+                           x.block, kBlock);
     if (x.steps_per_launch < 1 || x.steps_per_launch > 100000)
         return infoli_fail(INFOLI_ERR_ARG, "steps_per_launch %d out of range", x.steps_per_launch);
-    if (x.gather < GATHER_DIRECT || x.gather > GATHER_STAGED)
+    if (x.gather < GATHER_DIRECT || x.gather > GATHER_TILED)
         return infoli_fail(INFOLI_ERR_ARG, "infoli_set_exec: gather %d unknown", x.gather);
+    if (x.gather == GATHER_TILED && (x.tile_cells < 64 || x.tile_cells > 16384 || x.tile_cells % 64))
+        return infoli_fail(INFOLI_ERR_ARG, "tile_cells %d: must be a multiple of 64 in [64, 16384]",
+                           x.tile_cells);
+    if (x.gather == GATHER_TILED && x.mode == INFOLI_EXEC_PERSISTENT)
+        return infoli_fail(INFOLI_ERR_ARG, "the tiled gather runs as two kernels per step; "
+                                           "use the step or graph launch mode");
     if (x.gather == GATHER_VLDS && (size_t)c->n * c->tsize > kVldsMaxBytes)
         return infoli_fail(INFOLI_ERR_ARG, "gather LDS: %lld cells do not fit the LDS budget", (long long)c->n);
-    if (x.lds_kb < 8 || x.lds_kb > 156)
-        return infoli_fail(INFOLI_ERR_ARG, "lds_kb %d outside [8, 156]", x.lds_kb);
+    if (x.lds_kb < 1 || x.lds_kb > 156)
+        return infoli_fail(INFOLI_ERR_ARG, "lds_kb %d outside [1, 156]", x.lds_kb);
     if (x.gather == GATHER_STAGED && !max_block_entries_fit(c, x.lds_kb))
         return infoli_fail(INFOLI_ERR_ARG, "staged gather: a 64-cell group exceeds the %d KiB LDS budget",
                            x.lds_kb);
+    if (x.gather == GATHER_STAGED &&
+        ((int64_t)x.lds_kb * 1024 + (int64_t)(c->max_deg + 64 + kBlock) * (int64_t)c->tsize) > 160 * 1024)
+        return infoli_fail(INFOLI_ERR_ARG, "staged gather: %d KiB plus the row slack of degree %d exceeds "
+                                           "the 160 KiB of LDS", x.lds_kb, c->max_deg);
     const infoli_exec prev = c->exec;
     c->exec = x;
     drop_graph(c);
@@ -972,6 +1233,8 @@ int run_steps(infoli_ctx *c, StepArgs<T> a, int64_t t, int64_t count)
         return INFOLI_OK;
     }
     // persistent
+    if (c->exec.gather == GATHER_TILED)
+        return infoli_fail(INFOLI_ERR_ARG, "tiled gather has no persistent form");
     int grid = persist_grid(c);
     if (grid <= 0) return grid < 0 ? grid : infoli_fail(INFOLI_ERR_HIP, "no persistent grid");
     if (!c->profiling && count >= G) RCHK(capture_graph(c, a, mode, G, grid));
@@ -1055,10 +1318,9 @@ int advance_impl(infoli_ctx *c, int64_t n_steps, int64_t stride, double *trace)
     }
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, c->stream, c->d_flag);
-    const int64_t plane = c->npad;
     hipLaunchKernelGGL(k_check_finite<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
-                       (const T *)c->d_vd[c->step & 1], (const T *)c->d_state + V_SOMA * plane,
-                       (const T *)c->d_state + V_AXON * plane, n, c->d_flag);
+                       (const T *)c->d_vd[c->step & 1], (const T *)c->d_vs[c->step & 1],
+                       (const T *)c->d_va[c->step & 1], n, c->d_flag);
     HIPCHK(hipGetLastError());
     int flag = 0;
     GridBarrier gb{0, 0};
@@ -1154,10 +1416,10 @@ extern "C" int infoli_autotune(infoli_t *c, int64_t trial_steps, infoli_exec *ch
     HIPCHK(hipSetDevice(c->device));
     // save the full state so the trials leave no trace
     const size_t plane = (size_t)c->npad * c->tsize;
-    std::vector<unsigned char> saved(plane * (kNV + 1));
+    std::vector<unsigned char> saved(plane * kNV);
     HIPCHK(hipStreamSynchronize(c->stream));
-    HIPCHK(hipMemcpy(saved.data(), c->d_state, plane * kNV, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(saved.data() + plane * kNV, c->d_vd[c->step & 1], plane, hipMemcpyDeviceToHost));
+    for (int v = 0; v < kNV; v++)
+        HIPCHK(hipMemcpy(saved.data() + plane * v, var_ptr(c, v), plane, hipMemcpyDeviceToHost));
     const int64_t step0 = c->step;
     const infoli_exec orig = c->exec;
     const bool prof = c->profiling;
@@ -1170,12 +1432,16 @@ extern "C" int infoli_autotune(infoli_t *c, int64_t trial_steps, infoli_exec *ch
     const int spl = (int)std::min<int64_t>(trial_steps, 500);
     for (int L : Ls)
         for (int mode : modes) {
-            cands.push_back(infoli_exec{mode, L, kBlock, spl, GATHER_DIRECT, kDefaultLdsKb});
+            cands.push_back(infoli_exec{mode, L, kBlock, spl, GATHER_DIRECT, kDefaultLdsKb, 1024, 1});
             if ((size_t)c->n * c->tsize <= kVldsMaxBytes)
-                cands.push_back(infoli_exec{mode, L, kBlock, spl, GATHER_VLDS, kDefaultLdsKb});
+                cands.push_back(infoli_exec{mode, L, kBlock, spl, GATHER_VLDS, kDefaultLdsKb, 1024, 1});
             for (int kb : kbs)
                 if (max_block_entries_fit(c, kb))
-                    cands.push_back(infoli_exec{mode, L, kBlock, spl, GATHER_STAGED, kb});
+                    cands.push_back(infoli_exec{mode, L, kBlock, spl, GATHER_STAGED, kb, 1024, 1});
+            if (mode == INFOLI_EXEC_GRAPH && L <= 2)
+                for (int kb : {32, 48})
+                    for (int R : {1024, 2048})
+                        cands.push_back(infoli_exec{mode, L, 1024, spl, GATHER_TILED, kb, R, 1});
         }
     double best = 1e300;
     infoli_exec best_e = orig;
@@ -1201,8 +1467,8 @@ extern "C" int infoli_autotune(infoli_t *c, int64_t trial_steps, infoli_exec *ch
     c->exec = best_e;
     drop_graph(c);
     c->step = step0;
-    HIPCHK(hipMemcpy(c->d_state, saved.data(), plane * kNV, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d_vd[c->step & 1], saved.data() + plane * kNV, plane, hipMemcpyHostToDevice));
+    for (int v = 0; v < kNV; v++)
+        HIPCHK(hipMemcpy(var_ptr(c, v), saved.data() + plane * v, plane, hipMemcpyHostToDevice));
     c->profiling = prof;
     RCHK(ensure_layout(c));
     if (rc != INFOLI_OK) return infoli_fail(rc, "infoli_autotune: a trial failed (%s)", infoli_strerror(rc));

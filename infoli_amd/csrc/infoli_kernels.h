@@ -37,12 +37,22 @@ struct Model {
     T G_sd, G_ds, G_as, G_sa;
 };
 
+/* One tile of the 2D-tiled gap-junction kernel: the junctions from sender
+ * chunk `ch` into receiver block `rb`. */
+struct TileDesc {
+    int32_t rb, ch, nsl, pad;
+    int64_t sl0;               // first slice in tsp
+    int64_t rl0;               // first receiver slot in rlist / rdeg
+};
+
 template <typename T>
 struct StepArgs {
     int64_t n;            // cells
     int64_t npad;         // plane stride (multiple of kBlock)
     T *st;                // kNV planes of npad values
     T *vd[2];             // V_dend ping-pong: vd[t & 1] holds V_dend at the start of step t
+    T *vs[2];             // V_soma ping-pong (the dendrite and axon read the old value)
+    T *va[2];             // V_axon ping-pong
     const int32_t *deg;   // neighbour count per cell
     const int64_t *slice_ptr;
     const int32_t *col;   // sender ids, sliced-ELL
@@ -66,6 +76,17 @@ struct StepArgs {
     const uint32_t *gl32;      // (sender << sb) | slot, sorted by sender within a block
     const uint64_t *gl64;      // (sender << 32) | slot, when the packed form does not fit
     int sb;                    // slot bits of gl32
+    // 2D-tiled gap junctions (receiver block x sender chunk)
+    const TileDesc *tiles;     // tiles, receiver-block major, chunks ascending
+    const int32_t *rb_tile0;   // first tile of each receiver block, nrb + 1
+    const int64_t *tsp;        // per tile slice: first entry (64 entries per column)
+    const uint16_t *rlist;     // per tile: receivers (local to the block), in-chunk degree descending
+    const uint16_t *rdeg;      // in-chunk degree of each listed receiver
+    const uint16_t *tcol;      // sender index local to the chunk
+    const T *tw;               // conductance
+    void *part;                // partial sums, [tile][receiver block size] (sum, comp) pairs
+    int32_t rcells;            // receivers per block (multiple of 64)
+    int32_t ccells;            // senders per chunk
     Model<T> m;
 };
 
@@ -73,7 +94,9 @@ struct StepArgs {
 enum Gather {
     GATHER_DIRECT = 0,   // random loads from the V_dend array (L2-resident)
     GATHER_VLDS = 1,     // the whole V_dend array staged in LDS (small networks)
-    GATHER_STAGED = 2    // per block: senders sorted, copied to LDS, then read in row order
+    GATHER_STAGED = 2,   // per block: senders sorted, copied to LDS, then read in row order
+    GATHER_TILED = 3     // receiver block x sender chunk tiles, chunk of V_dend in LDS,
+                         // partial sums merged by the compartment kernel
 };
 
 struct GridBarrier {
@@ -83,6 +106,11 @@ struct GridBarrier {
 
 /* ---------------------------------------------------------------------- */
 /* compartment update, FP32: src/openmp/infoli.c:556-676                   */
+/*                                                                          */
+/* The three compartments are independent given the start-of-step voltages */
+/* (the dendrite reads V_s, the soma V_d and V_a, the axon V_s, all old),   */
+/* so each is a function of its own; `ode` runs all three.  Each returns    */
+/* its new voltage, committed by the caller (OMP:673-675).                  */
 /* ---------------------------------------------------------------------- */
 
 struct CellState32 {
@@ -95,9 +123,10 @@ struct CellState64 {
 __device__ __forceinline__ float fmin_ref(float a, float b) { return (a < b) ? a : b; }
 __device__ __forceinline__ double fmin_ref(double a, double b) { return (a < b) ? a : b; }
 
-__device__ __forceinline__ void ode(const Model<float> &P, CellState32 &c, float I_c, float iApp)
+/* dendrite, OMP:561-593; updates q, r, s, ca, icah */
+__device__ __forceinline__ float ode_dend(const Model<float> &P, CellState32 &c, float Vd, float Vs,
+                                          float I_c, float iApp)
 {
-    const float Vd = c.vd, Vs = c.vs, Va = c.va;
     // Dend H current OMP:561-564
     float q_inf = 1 / (1 + expf((Vd + 80) / 4));
     float tau_q = 1 / (expf(-0.086f * Vd - 14.6f) + expf(0.070f * Vd - 1.87f));
@@ -128,6 +157,12 @@ __device__ __forceinline__ void ode(const Model<float> &P, CellState32 &c, float
     float I_h = P.G_H * c.q * (Vd - P.V_H);
     float dVd_dt = (-(I_CaH + I_sd + I_ld + I_K_Ca + I_c + I_h) + iApp) / P.C_M;
     c.icah = I_CaH;
+    return P.DELTA * dVd_dt + Vd;
+}
+
+/* soma, OMP:599-642; updates k, l, m, h, n, p, xs */
+__device__ __forceinline__ float ode_soma(const Model<float> &P, CellState32 &c, float Vd, float Vs, float Va)
+{
     // Soma calcium OMP:599-606
     float k_inf = (1 / (1 + expf(-1 * (Vs + 61) / 4.2f)));
     float l_inf = (1 / (1 + expf((Vs + 85.5f) / 8.5f)));
@@ -169,6 +204,12 @@ __device__ __forceinline__ void ode(const Model<float> &P, CellState32 &c, float
     float I_K_s = P.G_K_S * (xs2 * xs2) * (Vs - P.V_K);
     float I_as = P.G_as * (Vs - Va);
     float dVs_dt = (-(I_CaL + I_ds + I_as + I_Na_s + I_ls + I_Kdr_s + I_K_s)) / P.C_M;
+    return P.DELTA * dVs_dt + Vs;
+}
+
+/* axon, OMP:648-668; updates ma, ha, xa */
+__device__ __forceinline__ float ode_axon(const Model<float> &P, CellState32 &c, float Vs, float Va)
+{
     // Axon sodium OMP:648-653
     float m_inf_a = 1 / (1 + (expf((-30 - Va) / 5.5f)));
     float h_inf_a = 1 / (1 + (expf((-60 - Va) / -5.8f)));
@@ -190,19 +231,18 @@ __device__ __forceinline__ void ode(const Model<float> &P, CellState32 &c, float
     float I_sa = P.G_sa * (Va - Vs);
     float I_K_a = P.G_K_A * (xa2 * xa2) * (Va - P.V_K);
     float dVa_dt = (-(I_K_a + I_sa + I_la + I_Na_a)) / P.C_M;
-    // new voltages OMP:673-675
-    c.vd = P.DELTA * dVd_dt + Vd;
-    c.vs = P.DELTA * dVs_dt + Vs;
-    c.va = P.DELTA * dVa_dt + Va;
+    return P.DELTA * dVa_dt + Va;
 }
 
 /* ---------------------------------------------------------------------- */
 /* compartment update, FP64: src/single_threaded/infoli.c:371-858          */
 /* ---------------------------------------------------------------------- */
 
-__device__ __forceinline__ void ode(const Model<double> &P, CellState64 &c, double I_c, double I_app)
+/* CompDend ST:371-428 (DendHCurr, DendCaCurr, DendKCurr, DendCal,
+ * DendCurrVolt) */
+__device__ __forceinline__ double ode_dend(const Model<double> &P, CellState64 &c, double prevV_dend,
+                                           double prevV_soma, double I_c, double I_app)
 {
-    const double prevV_dend = c.vd, prevV_soma = c.vs, prevV_axon = c.va;
     // DendHCurr ST:430-447
     double q_inf = 1 / (1 + exp((prevV_dend + 80) / 4));
     double tau_q = 1 / (exp(-0.086 * prevV_dend - 14.6) + exp(0.070 * prevV_dend - 1.87));
@@ -232,6 +272,14 @@ __device__ __forceinline__ void ode(const Model<double> &P, CellState64 &c, doub
     double I_ld = P.G_LD * (prevV_dend - P.V_L);
     double I_h = P.G_H * q * (prevV_dend - P.V_H);
     double dVd_dt = (-(I_CaH + I_sd + I_ld + I_K_Ca + I_c + I_h) + I_app) / P.C_M;
+    c.q = q; c.r = r; c.s = sK; c.ca = Ca; c.icah = I_CaH;
+    return P.DELTA * dVd_dt + prevV_dend;
+}
+
+/* CompSoma ST:560-616 */
+__device__ __forceinline__ double ode_soma(const Model<double> &P, CellState64 &c, double prevV_dend,
+                                           double prevV_soma, double prevV_axon)
+{
     // SomaCalcium ST:618-640
     double k_inf = (1 / (1 + exp(-1 * (prevV_soma + 61) / 4.2)));
     double l_inf = (1 / (1 + exp((prevV_soma + 85.5) / 8.5)));
@@ -272,6 +320,14 @@ __device__ __forceinline__ void ode(const Model<double> &P, CellState64 &c, doub
     double I_K_s = P.G_K_S * (xs2 * xs2) * (prevV_soma - P.V_K);
     double I_as = P.G_as * (prevV_soma - prevV_axon);
     double dVs_dt = (-(I_CaL + I_ds + I_as + I_Na_s + I_ls + I_Kdr_s + I_K_s)) / P.C_M;
+    c.k = k; c.l = l; c.m = m; c.h = h; c.n = n; c.p = p; c.xs = x_s;
+    return P.DELTA * dVs_dt + prevV_soma;
+}
+
+/* CompAxon ST:753-786 */
+__device__ __forceinline__ double ode_axon(const Model<double> &P, CellState64 &c, double prevV_soma,
+                                           double prevV_axon)
+{
     // AxonSodium ST:788-810
     double m_inf_a = 1 / (1 + (exp((-30 - prevV_axon) / 5.5)));
     double h_inf_a = 1 / (1 + (exp((-60 - prevV_axon) / -5.8)));
@@ -293,13 +349,21 @@ __device__ __forceinline__ void ode(const Model<double> &P, CellState64 &c, doub
     double xa2 = x_a * x_a;
     double I_K_a = P.G_K_A * (xa2 * xa2) * (prevV_axon - P.V_K);
     double dVa_dt = (-(I_K_a + I_sa + I_la + I_Na_a)) / P.C_M;
-
-    c.q = q; c.r = r; c.s = sK; c.ca = Ca; c.icah = I_CaH;
-    c.k = k; c.l = l; c.m = m; c.h = h; c.n = n; c.p = p; c.xs = x_s;
     c.ma = m_a; c.ha = h_a; c.xa = x_a;
-    c.vd = P.DELTA * dVd_dt + prevV_dend;
-    c.vs = P.DELTA * dVs_dt + prevV_soma;
-    c.va = P.DELTA * dVa_dt + prevV_axon;
+    return P.DELTA * dVa_dt + prevV_axon;
+}
+
+/* all three compartments, voltages committed last */
+template <typename T, typename C>
+__device__ __forceinline__ void ode(const Model<T> &P, C &c, T I_c, T iApp)
+{
+    const T Vd = c.vd, Vs = c.vs, Va = c.va;
+    const T vd = ode_dend(P, c, Vd, Vs, I_c, iApp);
+    const T vs = ode_soma(P, c, Vd, Vs, Va);
+    const T va = ode_axon(P, c, Vs, Va);
+    c.vd = vd;
+    c.vs = vs;
+    c.va = va;
 }
 
 template <typename T> struct CellOf;
@@ -366,7 +430,7 @@ __device__ __forceinline__ float shfl_xor(float v, int m) { return __shfl_xor(v,
 __device__ __forceinline__ double shfl_xor(double v, int m) { return __shfl_xor(v, m, 64); }
 
 /* ---------------------------------------------------------------------- */
-/* one workgroup, one block of cells, one step                              */
+/* cell state in HBM                                                        */
 /* ---------------------------------------------------------------------- */
 
 template <typename T>
@@ -378,6 +442,130 @@ __device__ __forceinline__ T iapp_of(const StepArgs<T> &a, int64_t cell, int64_t
     bool on = (t >= a.t0) && (t < a.t1) && (!a.mask || a.mask[cell]);
     return on ? b + a.amp : b;
 }
+
+/* the start-of-step state of a cell: voltages from the ping-pong halves
+ * of step t, gates from the in-place planes */
+template <typename T, typename C>
+__device__ __forceinline__ void load_cell(const StepArgs<T> &a, int64_t cell, int64_t t, C &c)
+{
+    const int64_t np = a.npad;
+    const T *st = a.st;
+    c.vd = a.vd[t & 1][cell];
+    c.vs = a.vs[t & 1][cell];
+    c.va = a.va[t & 1][cell];
+    c.r = st[CALCIUM_R * np + cell];
+    c.s = st[POTASSIUM_S * np + cell];
+    c.q = st[HCURRENT_Q * np + cell];
+    c.ca = st[CA2PLUS * np + cell];
+    c.icah = st[I_CAH * np + cell];
+    c.gcal = st[G_CAL * np + cell];
+    c.h = st[SODIUM_H * np + cell];
+    c.n = st[POTASSIUM_N * np + cell];
+    c.p = st[POTASSIUM_P * np + cell];
+    c.xs = st[POTASSIUM_X_S * np + cell];
+    c.k = st[CALCIUM_K * np + cell];
+    c.l = st[CALCIUM_L * np + cell];
+    c.ha = st[SODIUM_H_A * np + cell];
+    c.xa = st[POTASSIUM_X_A * np + cell];
+    c.m = 0;
+    c.ma = 0;
+}
+
+template <typename T, typename C>
+__device__ __forceinline__ void store_cell(const StepArgs<T> &a, int64_t cell, int64_t t, const C &c)
+{
+    const int64_t np = a.npad;
+    T *st = a.st;
+    a.vd[(t & 1) ^ 1][cell] = c.vd;
+    a.vs[(t & 1) ^ 1][cell] = c.vs;
+    a.va[(t & 1) ^ 1][cell] = c.va;
+    st[CALCIUM_R * np + cell] = c.r;
+    st[POTASSIUM_S * np + cell] = c.s;
+    st[HCURRENT_Q * np + cell] = c.q;
+    st[CA2PLUS * np + cell] = c.ca;
+    st[I_CAH * np + cell] = c.icah;
+    st[SODIUM_M * np + cell] = c.m;
+    st[SODIUM_H * np + cell] = c.h;
+    st[POTASSIUM_N * np + cell] = c.n;
+    st[POTASSIUM_P * np + cell] = c.p;
+    st[POTASSIUM_X_S * np + cell] = c.xs;
+    st[CALCIUM_K * np + cell] = c.k;
+    st[CALCIUM_L * np + cell] = c.l;
+    st[SODIUM_M_A * np + cell] = c.ma;
+    st[SODIUM_H_A * np + cell] = c.ha;
+    st[POTASSIUM_X_A * np + cell] = c.xa;
+    if (a.trace && (t % a.stride) == 0)
+        a.trace[((t / a.stride) % a.trace_cap) * a.n + cell] = c.va;
+}
+
+/* ---------------------------------------------------------------------- */
+/* one lane's share of a sliced-ELL row                                     */
+/* ---------------------------------------------------------------------- */
+
+constexpr int kUnroll = 8;   // row entries in flight per lane
+
+/* The lane owns row positions p = kk*L + l (kk = 0, 1, ...) of a row of d
+ * entries; entry kk sits at base + kk*64.  W = the slice width, uniform
+ * across the wavefront, so every slot kk < W is allocated: slots past the
+ * row's end are padding (sender 0, conductance 0) and are loaded like the
+ * others -- no divergent branches, kUnroll loads in flight -- while their
+ * term is replaced by an exact +0 (a no-op for the plain and the TwoSum
+ * accumulator).  Terms are accumulated in position order. */
+template <typename T, bool KAHAN, int L, typename VOf>
+__device__ __forceinline__ void row_terms(Acc<T, KAHAN> &acc, T vi, int d, int l, int W, int64_t base,
+                                          const T *__restrict__ w, VOf v_of)
+{
+    for (int kk0 = 0; kk0 < W; kk0 += kUnroll) {
+        T wv[kUnroll], vv[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            if (kk0 + u < W) {
+                wv[u] = w[base + (int64_t)(kk0 + u) * 64];
+                vv[u] = v_of(kk0 + u);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            if (kk0 + u < W) {
+                const T term = gj_term(vi, vv[u], wv[u]);
+                acc.add(((kk0 + u) * L + l < d) ? term : T(0));
+            }
+        }
+    }
+}
+
+/* the same with the sender index loaded first (direct and tiled gathers) */
+template <typename T, bool KAHAN, int L, typename IdxT, typename VOf>
+__device__ __forceinline__ void row_terms_idx(Acc<T, KAHAN> &acc, T vi, int d, int l, int W, int64_t base,
+                                              const T *__restrict__ w, const IdxT *__restrict__ idx, VOf v_of)
+{
+    for (int kk0 = 0; kk0 < W; kk0 += kUnroll) {
+        int jv[kUnroll];
+        T wv[kUnroll], vv[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            if (kk0 + u < W) {
+                const int64_t e = base + (int64_t)(kk0 + u) * 64;
+                jv[u] = (int)idx[e];
+                wv[u] = w[e];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++)
+            if (kk0 + u < W) vv[u] = v_of(jv[u]);
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            if (kk0 + u < W) {
+                const T term = gj_term(vi, vv[u], wv[u]);
+                acc.add(((kk0 + u) * L + l < d) ? term : T(0));
+            }
+        }
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+/* one workgroup, one block of cells, one step                              */
+/* ---------------------------------------------------------------------- */
 
 /* Phase 1 of the staged gather: the block's neighbour entries, sorted by
  * sender id, copy V_dend[sender] into LDS slot `slot` (the entry's position
@@ -394,13 +582,16 @@ __device__ __forceinline__ void stage_gather(const StepArgs<T> &a, int64_t blk, 
         const unsigned sb = (unsigned)a.sb;
         const uint32_t mask = (1u << sb) - 1u;
         int64_t e = g0 + tid;
-        for (; e + 3 * kBlock < g1; e += 4 * kBlock) {
-            const uint32_t u0 = gl[e], u1 = gl[e + kBlock], u2 = gl[e + 2 * kBlock], u3 = gl[e + 3 * kBlock];
-            const T x0 = vin[u0 >> sb], x1 = vin[u1 >> sb], x2 = vin[u2 >> sb], x3 = vin[u3 >> sb];
-            vj[u0 & mask] = x0;
-            vj[u1 & mask] = x1;
-            vj[u2 & mask] = x2;
-            vj[u3 & mask] = x3;
+        constexpr int GU = 8;
+        for (; e + (GU - 1) * kBlock < g1; e += GU * kBlock) {
+            uint32_t u[GU];
+            T x[GU];
+#pragma unroll
+            for (int q = 0; q < GU; q++) u[q] = gl[e + q * kBlock];
+#pragma unroll
+            for (int q = 0; q < GU; q++) x[q] = vin[u[q] >> sb];
+#pragma unroll
+            for (int q = 0; q < GU; q++) vj[u[q] & mask] = x[q];
         }
         for (; e < g1; e += kBlock) {
             const uint32_t u = gl[e];
@@ -424,7 +615,6 @@ __device__ __forceinline__ void step_block(const StepArgs<T> &a, int64_t blk, in
     const int lane = tid & 63, wave = tid >> 6;
     const int cw = lane / L, l = lane % L;
     const T *__restrict__ vin = a.vd[t & 1];
-    T *__restrict__ vout = a.vd[(t & 1) ^ 1];
     int64_t cell0;
     int ncell;
     if constexpr (G == GATHER_STAGED) {
@@ -455,33 +645,18 @@ __device__ __forceinline__ void step_block(const StepArgs<T> &a, int64_t blk, in
             const int d = a.deg[cell];
             if (d > 0) {
                 const T vi = (G == GATHER_VLDS) ? dyn[cell] : vin[cell];
-                const int64_t base = a.slice_ptr[cell0 / CPW + sl] + cw * L + l;
-                const int nk = (d > l) ? (d - l + L - 1) / L : 0;
-                int kk = 0;
+                const int64_t s_ = cell0 / CPW + sl;
+                const int64_t base0 = a.slice_ptr[s_];
+                const int W = (int)((a.slice_ptr[s_ + 1] - base0) >> 6);
+                const int64_t base = base0 + cw * L + l;
                 if constexpr (G == GATHER_STAGED) {
                     const T *__restrict__ vj = dyn + a.lrs[cell] + l;
-                    for (; kk + 2 <= nk; kk += 2) {
-                        const int64_t e0 = base + (int64_t)kk * 64;
-                        const T w0 = a.w[e0], w1 = a.w[e0 + 64];
-                        const T v0 = vj[kk * L], v1 = vj[(kk + 1) * L];
-                        acc.add(gj_term(vi, v0, w0));
-                        acc.add(gj_term(vi, v1, w1));
-                    }
-                    if (kk < nk) acc.add(gj_term(vi, vj[kk * L], a.w[base + (int64_t)kk * 64]));
+                    row_terms<T, KAHAN, L>(acc, vi, d, l, W, base, a.w,
+                                           [&](int kk) { return vj[kk * L]; });
                 } else {
                     const T *__restrict__ vsrc = (G == GATHER_VLDS) ? dyn : vin;
-                    for (; kk + 2 <= nk; kk += 2) {
-                        const int64_t e0 = base + (int64_t)kk * 64, e1 = e0 + 64;
-                        const int j0 = a.col[e0], j1 = a.col[e1];
-                        const T w0 = a.w[e0], w1 = a.w[e1];
-                        const T v0 = vsrc[j0], v1 = vsrc[j1];
-                        acc.add(gj_term(vi, v0, w0));
-                        acc.add(gj_term(vi, v1, w1));
-                    }
-                    if (kk < nk) {
-                        const int64_t e0 = base + (int64_t)kk * 64;
-                        acc.add(gj_term(vi, vsrc[a.col[e0]], a.w[e0]));
-                    }
+                    row_terms_idx<T, KAHAN, L>(acc, vi, d, l, W, base, a.w, a.col,
+                                               [&](int j) { return vsrc[j]; });
                 }
             }
         }
@@ -500,48 +675,10 @@ __device__ __forceinline__ void step_block(const StepArgs<T> &a, int64_t blk, in
     // (2) compartment update, one thread per cell
     const int64_t cell = cell0 + tid;
     if (tid < ncell && cell < a.n) {
-        const int64_t np = a.npad;
-        T *st = a.st;
         typename CellOf<T>::type c;
-        c.vd = vin[cell];
-        c.r = st[CALCIUM_R * np + cell];
-        c.s = st[POTASSIUM_S * np + cell];
-        c.q = st[HCURRENT_Q * np + cell];
-        c.ca = st[CA2PLUS * np + cell];
-        c.icah = st[I_CAH * np + cell];
-        c.gcal = st[G_CAL * np + cell];
-        c.vs = st[V_SOMA * np + cell];
-        c.h = st[SODIUM_H * np + cell];
-        c.n = st[POTASSIUM_N * np + cell];
-        c.p = st[POTASSIUM_P * np + cell];
-        c.xs = st[POTASSIUM_X_S * np + cell];
-        c.k = st[CALCIUM_K * np + cell];
-        c.l = st[CALCIUM_L * np + cell];
-        c.va = st[V_AXON * np + cell];
-        c.ha = st[SODIUM_H_A * np + cell];
-        c.xa = st[POTASSIUM_X_A * np + cell];
-        c.m = 0; c.ma = 0;
+        load_cell(a, cell, t, c);
         ode(a.m, c, ic_lds[tid], iapp_of(a, cell, t));
-        vout[cell] = c.vd;
-        st[CALCIUM_R * np + cell] = c.r;
-        st[POTASSIUM_S * np + cell] = c.s;
-        st[HCURRENT_Q * np + cell] = c.q;
-        st[CA2PLUS * np + cell] = c.ca;
-        st[I_CAH * np + cell] = c.icah;
-        st[V_SOMA * np + cell] = c.vs;
-        st[SODIUM_M * np + cell] = c.m;
-        st[SODIUM_H * np + cell] = c.h;
-        st[POTASSIUM_N * np + cell] = c.n;
-        st[POTASSIUM_P * np + cell] = c.p;
-        st[POTASSIUM_X_S * np + cell] = c.xs;
-        st[CALCIUM_K * np + cell] = c.k;
-        st[CALCIUM_L * np + cell] = c.l;
-        st[V_AXON * np + cell] = c.va;
-        st[SODIUM_M_A * np + cell] = c.ma;
-        st[SODIUM_H_A * np + cell] = c.ha;
-        st[POTASSIUM_X_A * np + cell] = c.xa;
-        if (a.trace && (t % a.stride) == 0)
-            a.trace[((t / a.stride) % a.trace_cap) * a.n + cell] = c.va;
+        store_cell(a, cell, t, c);
     }
     __syncthreads();   // ic_lds / dyn are reused by the next block or step
 }
@@ -646,6 +783,175 @@ __global__ __launch_bounds__(kBlock) void k_gap_currents(StepArgs<T> a, int64_t 
             }
         }
         if (l == 0 && cell < a.n) out[cell] = acc.value();
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+/* 2D-tiled gap junctions + separate compartment kernel                     */
+/* ---------------------------------------------------------------------- */
+
+template <typename T, bool KAHAN> struct PartOf { using type = T; };
+template <> struct PartOf<float, true> { using type = float2; };
+
+template <typename T, bool KAHAN>
+__device__ __forceinline__ void part_store(void *p, int64_t i, const Acc<T, KAHAN> &a)
+{
+    if constexpr (KAHAN) reinterpret_cast<float2 *>(p)[i] = make_float2(a.s, a.c);
+    else reinterpret_cast<T *>(p)[i] = a.s;
+}
+
+template <typename T, bool KAHAN>
+__device__ __forceinline__ void part_merge(const void *p, int64_t i, Acc<T, KAHAN> &a)
+{
+    if constexpr (KAHAN) {
+        const float2 v = reinterpret_cast<const float2 *>(p)[i];
+        a.merge(v.x, v.y);
+    } else {
+        a.merge(reinterpret_cast<const T *>(p)[i], T(0));
+    }
+}
+
+/* One tile: copy the sender chunk's V_dend to LDS, then each receiver's
+ * junctions from that chunk (L lanes per receiver, list order) into a
+ * partial (sum, compensation) pair. */
+template <typename T, int L, bool KAHAN, int NT>
+__global__ __launch_bounds__(NT) void k_tile_gj(StepArgs<T> a, int64_t k)
+{
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
+    T *vch = reinterpret_cast<T *>(dyn_lds);
+    constexpr int CPW = 64 / L;
+    const int64_t t = (a.d_step ? *a.d_step : 0) + k;
+    const T *__restrict__ vin = a.vd[t & 1];
+    const TileDesc td = a.tiles[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int cw = lane / L, l = lane % L;
+    const int64_t s0 = (int64_t)td.ch * a.ccells;
+    const int64_t cn = (a.n - s0 < a.ccells) ? a.n - s0 : a.ccells;
+    for (int64_t i = tid; i < cn; i += NT) vch[i] = vin[s0 + i];
+    __syncthreads();
+    const int64_t cellb = (int64_t)td.rb * a.rcells;
+    for (int sl = wave; sl < td.nsl; sl += NT / 64) {
+        const int64_t slot = td.rl0 + (int64_t)sl * CPW + cw;
+        const int rloc = a.rlist[slot];
+        const int d = a.rdeg[slot];
+        const int64_t cell = cellb + rloc;
+        Acc<T, KAHAN> acc;
+        if (d > 0) {
+            const T vi = vin[cell];
+            const int64_t base0 = a.tsp[td.sl0 + sl];
+            const int W = (int)((a.tsp[td.sl0 + sl + 1] - base0) >> 6);
+            row_terms_idx<T, KAHAN, L>(acc, vi, d, l, W, base0 + cw * L + l, a.tw, a.tcol,
+                                       [&](int j) { return vch[j]; });
+        }
+        if constexpr (L > 1) {
+#pragma unroll
+            for (int off = L / 2; off > 0; off >>= 1) {
+                T os = shfl_xor(acc.s, off);
+                T oc = KAHAN ? shfl_xor(acc.c, off) : T(0);
+                acc.merge(os, oc);
+            }
+        }
+        if (l == 0) part_store<T, KAHAN>(a.part, (int64_t)blockIdx.x * a.rcells + rloc, acc);
+    }
+}
+
+/* I_c of a cell: its partials over the tiles of its receiver block, merged
+ * in ascending sender-chunk order. */
+template <typename T, bool KAHAN>
+__device__ __forceinline__ T tiled_ic(const StepArgs<T> &a, int64_t cell)
+{
+    using P = typename PartOf<T, KAHAN>::type;
+    const P *__restrict__ part = reinterpret_cast<const P *>(a.part);
+    const int64_t rb = cell / a.rcells, rloc = cell - rb * a.rcells;
+    const int32_t t0 = a.rb_tile0[rb], t1 = a.rb_tile0[rb + 1];   // uniform per wavefront
+    Acc<T, KAHAN> acc;
+    constexpr int B = 8;
+    for (int32_t tt = t0; tt < t1; tt += B) {
+        P v[B];
+#pragma unroll
+        for (int q = 0; q < B; q++)
+            if (tt + q < t1) v[q] = part[(int64_t)(tt + q) * a.rcells + rloc];
+#pragma unroll
+        for (int q = 0; q < B; q++) {
+            if (tt + q < t1) {
+                if constexpr (KAHAN) acc.merge(v[q].x, v[q].y);
+                else acc.merge(v[q], T(0));
+            }
+        }
+    }
+    return acc.value();
+}
+
+/* compartment update of every cell, I_c from the tile partials */
+template <typename T, bool KAHAN>
+__global__ __launch_bounds__(kBlock) void k_tile_ode(StepArgs<T> a, int64_t k)
+{
+    const int64_t t = (a.d_step ? *a.d_step : 0) + k;
+    const int64_t cell = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (cell >= a.n) return;
+    typename CellOf<T>::type c;
+    load_cell(a, cell, t, c);
+    ode(a.m, c, tiled_ic<T, KAHAN>(a, cell), iapp_of(a, cell, t));
+    store_cell(a, cell, t, c);
+}
+
+/* The same update with one wavefront per compartment: a workgroup of 192
+ * threads covers 64 cells; wave 0 runs the dendrites, wave 1 the somata,
+ * wave 2 the axons, each from the start-of-step voltages. */
+template <typename T, bool KAHAN>
+__global__ __launch_bounds__(192) void k_tile_ode3(StepArgs<T> a, int64_t k)
+{
+    const int64_t t = (a.d_step ? *a.d_step : 0) + k;
+    const int wave = threadIdx.x >> 6;
+    const int64_t cell = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
+    if (cell >= a.n) return;
+    const int64_t np = a.npad;
+    T *st = a.st;
+    const int cur = (int)(t & 1), nxt = cur ^ 1;
+    const T Vd = a.vd[cur][cell], Vs = a.vs[cur][cell];
+    typename CellOf<T>::type c;
+    if (wave == 0) {
+        c.q = st[HCURRENT_Q * np + cell];
+        c.r = st[CALCIUM_R * np + cell];
+        c.s = st[POTASSIUM_S * np + cell];
+        c.ca = st[CA2PLUS * np + cell];
+        c.icah = st[I_CAH * np + cell];
+        const T vd = ode_dend(a.m, c, Vd, Vs, tiled_ic<T, KAHAN>(a, cell), iapp_of(a, cell, t));
+        st[HCURRENT_Q * np + cell] = c.q;
+        st[CALCIUM_R * np + cell] = c.r;
+        st[POTASSIUM_S * np + cell] = c.s;
+        st[CA2PLUS * np + cell] = c.ca;
+        st[I_CAH * np + cell] = c.icah;
+        a.vd[nxt][cell] = vd;
+    } else if (wave == 1) {
+        const T Va = a.va[cur][cell];
+        c.k = st[CALCIUM_K * np + cell];
+        c.l = st[CALCIUM_L * np + cell];
+        c.h = st[SODIUM_H * np + cell];
+        c.n = st[POTASSIUM_N * np + cell];
+        c.p = st[POTASSIUM_P * np + cell];
+        c.xs = st[POTASSIUM_X_S * np + cell];
+        c.gcal = st[G_CAL * np + cell];
+        const T vs = ode_soma(a.m, c, Vd, Vs, Va);
+        st[CALCIUM_K * np + cell] = c.k;
+        st[CALCIUM_L * np + cell] = c.l;
+        st[SODIUM_M * np + cell] = c.m;
+        st[SODIUM_H * np + cell] = c.h;
+        st[POTASSIUM_N * np + cell] = c.n;
+        st[POTASSIUM_P * np + cell] = c.p;
+        st[POTASSIUM_X_S * np + cell] = c.xs;
+        a.vs[nxt][cell] = vs;
+    } else {
+        const T Va = a.va[cur][cell];
+        c.ha = st[SODIUM_H_A * np + cell];
+        c.xa = st[POTASSIUM_X_A * np + cell];
+        const T va = ode_axon(a.m, c, Vs, Va);
+        st[SODIUM_M_A * np + cell] = c.ma;
+        st[SODIUM_H_A * np + cell] = c.ha;
+        st[POTASSIUM_X_A * np + cell] = c.xa;
+        a.va[nxt][cell] = va;
+        if (a.trace && (t % a.stride) == 0)
+            a.trace[((t / a.stride) % a.trace_cap) * a.n + cell] = va;
     }
 }
 

@@ -17,7 +17,7 @@ FP64, FP32, FP32_KAHAN = 0, 1, 2
 _PREC_NAMES = {"fp64": FP64, "fp32": FP32, "fp32k": FP32_KAHAN, "fp32_kahan": FP32_KAHAN}
 
 EXEC_AUTO, EXEC_STEP, EXEC_GRAPH, EXEC_PERSISTENT, EXEC_BATCHED = 0, 1, 2, 3, 4
-GATHER_DIRECT, GATHER_LDS, GATHER_STAGED = 0, 1, 2
+GATHER_DIRECT, GATHER_LDS, GATHER_STAGED, GATHER_TILED = 0, 1, 2, 3
 
 # per-cell variables, bin/default.conf lines 24-42 order
 VARS = ("V_dend", "Calcium_r", "Potassium_s", "Hcurrent_q", "Ca2Plus", "I_CaH",
@@ -42,7 +42,8 @@ class _Params(ctypes.Structure):
 class _Exec(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("lanes_per_cell", ctypes.c_int),
                 ("block", ctypes.c_int), ("steps_per_launch", ctypes.c_int),
-                ("gather", ctypes.c_int), ("lds_kb", ctypes.c_int)]
+                ("gather", ctypes.c_int), ("lds_kb", ctypes.c_int),
+                ("tile_cells", ctypes.c_int), ("ode_split", ctypes.c_int)]
 
 
 _lib = None
@@ -288,14 +289,15 @@ class Network:
         e = _Exec()
         _check(library().infoli_get_exec(self._h, ctypes.byref(e)), "get_exec")
         return dict(mode=e.mode, lanes_per_cell=e.lanes_per_cell, block=e.block,
-                    steps_per_launch=e.steps_per_launch, gather=e.gather, lds_kb=e.lds_kb)
+                    steps_per_launch=e.steps_per_launch, gather=e.gather, lds_kb=e.lds_kb,
+                    tile_cells=e.tile_cells, ode_split=e.ode_split)
 
     def set_exec(self, mode=None, lanes_per_cell=None, block=None, steps_per_launch=None,
-                 gather=None, lds_kb=None):
+                 gather=None, lds_kb=None, tile_cells=None, ode_split=None):
         cur = self.exec
         for k, v in (("mode", mode), ("lanes_per_cell", lanes_per_cell), ("block", block),
                      ("steps_per_launch", steps_per_launch), ("gather", gather),
-                     ("lds_kb", lds_kb)):
+                     ("lds_kb", lds_kb), ("tile_cells", tile_cells), ("ode_split", ode_split)):
             if v is not None:
                 cur[k] = int(v)
         e = _Exec(**cur)

@@ -100,7 +100,8 @@ MODES = [(I.EXEC_STEP, "step"), (I.EXEC_GRAPH, "graph"), (I.EXEC_PERSISTENT, "pe
 
 GATHERS = [(I.GATHER_DIRECT, 76, "direct"), (I.GATHER_LDS, 76, "lds"),
            (I.GATHER_STAGED, 8, "staged8k"), (I.GATHER_STAGED, 76, "staged76k"),
-           (I.GATHER_STAGED, 150, "staged150k")]
+           (I.GATHER_STAGED, 150, "staged150k"), (I.GATHER_TILED, 1, "tiled1k"),
+           (I.GATHER_TILED, 76, "tiled76k")]
 
 
 @pytest.mark.parametrize("mode,mname", MODES)
@@ -116,7 +117,19 @@ def test_exec_strategies_agree(mode, mname, L, gather, kb, gname, prec):
         # 64-cell group fits, giving many small blocks
         rp, col, w, gcal, iapp = _random_case(n, p=0.02, seed=9)
     net = make_net(n, prec, rp, col, w, gcal, iapp)
-    net.set_exec(mode=mode, lanes_per_cell=L, gather=gather, lds_kb=kb, steps_per_launch=50)
+    if gather == I.GATHER_TILED:
+        if mode == I.EXEC_PERSISTENT:
+            with pytest.raises(I.InfoliError):
+                net.set_exec(mode=mode, gather=gather)
+            net.close()
+            return
+        # 1 KiB chunks (256 FP32 / 128 FP64 senders) with 128-cell receiver
+        # blocks: several tiles per block, partials merged across chunks
+        net.set_exec(mode=mode, lanes_per_cell=L, gather=gather, lds_kb=kb,
+                     tile_cells=128 if kb == 1 else 1024, steps_per_launch=50,
+                     block=256 if kb == 1 else 1024, ode_split=0 if kb == 1 else 1)
+    else:
+        net.set_exec(mode=mode, lanes_per_cell=L, gather=gather, lds_kb=kb, steps_per_launch=50)
     out = net.advance(steps, trace_stride=10)
     vd = net.get_state("V_dend")
     net.close()
