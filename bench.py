@@ -27,6 +27,22 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+GATHER_NAMES = {0: "direct", 1: "lds", 2: "staged", 3: "tiled"}
+KERNELS = {0: "k_step", 1: "k_step", 2: "k_step", 3: "k_tile_gj+k_tile_ode3"}
+
+
+def measured_traffic(kernel, workload, prec):
+    """HBM bytes per step from the committed rocprofv3 --pmc summary of this
+    kernel and workload (FETCH_SIZE and WRITE_SIZE scaled by a copy kernel
+    of known size), or None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_{prec}.md")
+    try:
+        for line in open(path):
+            if line.startswith("| per step |") and "total" in line:
+                return float(line.split("total")[1].split("MB")[0]) * 1e6
+    except OSError:
+        pass
+    return None
 METRIC = "cell-steps/s at 100k cells, FP32+Kahan batched vs FP64 plain; % roofline"
 
 
@@ -150,6 +166,9 @@ def main():
     achieved = bytes_per_launch / (mean_launch_ms * 1e-3) / 1e9
     net.close()
 
+    kname = KERNELS[exec_cfg["gather"]]
+    tr = measured_traffic(kname, wl["name"], args.precision)
+    traffic = None if tr is None else tr * steps_per_launch
     extra = {"device_ms": round(dev_ms, 3), "exec": exec_cfg,
              "mean_kernel_launch_ms": round(mean_launch_ms, 5),
              "steps_per_kernel_launch": steps_per_launch}
@@ -187,9 +206,10 @@ def main():
                        "fp64": "fp64"}[args.precision],
                    "trace_stride": args.stride, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_launch": bytes_per_launch,
-                     "kernel": "k_step (fused gap-junction + compartment step)"},
+                     "launch": "one step" if steps_per_launch == 1 else f"{steps_per_launch:g} steps",
+                     "kernel": kname},
         "cpu_baseline": cpu,
         "extra": extra,
     }

@@ -199,6 +199,9 @@ typedef struct infoli_exec {
     int     lds_kb;          /* staged / tiled: LDS budget per workgroup, KiB */
     int     tile_cells;      /* tiled: receivers per tile (multiple of 64)    */
     int     ode_split;       /* tiled: one wavefront per compartment (0/1)    */
+    int     fast_math;       /* tiled, FP32: hardware exp2 and reciprocal in
+                                the compartment update (0 = libm expf and IEEE
+                                division, the reference's arithmetic)        */
 } infoli_exec;
 
 int infoli_get_exec(const infoli_t *ctx, infoli_exec *out);

@@ -275,13 +275,18 @@ const void *const k_tile_gj_tab[3][3][5] = {
     INFOLI_NT_TABLE(float, true),
 };
 int ntidx(int nt) { return nt == 1024 ? 2 : nt == 512 ? 1 : 0; }
-const void *const k_tile_ode_tab[3] = {
-    (const void *)k_tile_ode<double, false>, (const void *)k_tile_ode<float, false>,
-    (const void *)k_tile_ode<float, true>,
+/* [fast math][precision] */
+const void *const k_tile_ode_tab[2][3] = {
+    {(const void *)k_tile_ode<double, false, false>, (const void *)k_tile_ode<float, false, false>,
+     (const void *)k_tile_ode<float, true, false>},
+    {(const void *)k_tile_ode<double, false, false>, (const void *)k_tile_ode<float, false, true>,
+     (const void *)k_tile_ode<float, true, true>},
 };
-const void *const k_tile_ode3_tab[3] = {
-    (const void *)k_tile_ode3<double, false>, (const void *)k_tile_ode3<float, false>,
-    (const void *)k_tile_ode3<float, true>,
+const void *const k_tile_ode3_tab[2][3] = {
+    {(const void *)k_tile_ode3<double, false, false>, (const void *)k_tile_ode3<float, false, false>,
+     (const void *)k_tile_ode3<float, true, false>},
+    {(const void *)k_tile_ode3<double, false, false>, (const void *)k_tile_ode3<float, false, true>,
+     (const void *)k_tile_ode3<float, true, true>},
 };
 const void *const k_gap_tab[3][5] = {
     INFOLI_L_TABLE(k_gap_currents, double, false),
@@ -317,14 +322,15 @@ int launch_step(infoli_ctx *c, const void *args, int64_t k)
 {
     if (c->exec.gather == GATHER_TILED) {
         const void *fg = k_tile_gj_tab[c->prec][ntidx(c->exec.block)][lidx(c->exec.lanes_per_cell)];
-        const void *fo = k_tile_ode_tab[c->prec];
+        const int fm = c->exec.fast_math ? 1 : 0;
+        const void *fo = k_tile_ode_tab[fm][c->prec];
         int64_t kk = k;
         void *kargs[] = {const_cast<void *>(args), &kk};
         const size_t lds = (size_t)c->tl_C * c->tsize;
         allow_dyn_lds(fg, lds);
         HIPCHK(hipLaunchKernel(fg, dim3((unsigned)c->tl_ntiles), dim3(c->exec.block), kargs, lds, c->stream));
         if (c->exec.ode_split)
-            HIPCHK(hipLaunchKernel(k_tile_ode3_tab[c->prec], dim3((unsigned)(c->npad / 64)), dim3(192), kargs, 0,
+            HIPCHK(hipLaunchKernel(k_tile_ode3_tab[fm][c->prec], dim3((unsigned)(c->npad / 64)), dim3(192), kargs, 0,
                                    c->stream));
         else
             HIPCHK(hipLaunchKernel(fo, dim3((unsigned)(c->npad / kBlock)), dim3(kBlock), kargs, 0, c->stream));
@@ -502,7 +508,13 @@ int upload_tiled(infoli_ctx *c, int L, int R, int kb)
             td.rb = (int32_t)rb;
             td.ch = (int32_t)ch;
             td.nsl = R / CPW;
-            td.pad = 0;
+            // constant conductance over the tile (in the precision of the step)
+            const T w0 = (T)c->w[ent[ch][0]];
+            bool cw_all = true;
+            for (int64_t e : ent[ch])
+                if ((T)c->w[e] != w0) { cw_all = false; break; }
+            td.wconst = cw_all ? 1 : 0;
+            td.wc = (double)w0;
             td.sl0 = (int64_t)tsp.size() - 1;
             td.rl0 = (int64_t)rlist.size();
             for (int r : rs) {
@@ -535,7 +547,7 @@ int upload_tiled(infoli_ctx *c, int L, int R, int kb)
     }
     rbt0[nrb] = (int32_t)tiles.size();
     if (tiles.empty()) {   // no junctions at all: one empty tile keeps the kernels well-formed
-        TileDesc td{0, 0, 0, 0, 0, 0};
+        TileDesc td{0, 0, 0, 0, 0, 0, 0.0};
         tiles.push_back(td);
         std::fill(rbt0.begin(), rbt0.end(), 0);
     }
@@ -733,16 +745,31 @@ void default_exec(infoli_ctx *c)
     c->exec.lds_kb = kDefaultLdsKb;
     c->exec.tile_cells = 2048;
     c->exec.ode_split = 1;
+    c->exec.fast_math = 0;
     c->exec.gather = GATHER_DIRECT;
     if (mean_deg < 4) return;
-    const int tkb = 48;
-    const int64_t C = tiled_chunk(c, tkb);
+    // tiled: lanes per receiver from the mean in-tile degree of a first
+    // layout guess, receivers per tile inversely, then sender chunks small
+    // enough for at least one tile per CU
+    const int64_t C0 = tiled_chunk(c, 48);
+    const int R0 = 2048;
+    const int64_t nrb0 = (c->n + R0 - 1) / R0;
+    const double tiles0 = tiles_per_block(c, R0, C0) * (double)nrb0;
+    const double dc = tiles0 > 0 ? (double)c->col.size() / (tiles0 * R0) : 0.0;
+    int L = 1;
+    while (L < 16 && 2.0 * L * 16.0 <= dc) L *= 2;
+    const int R = std::max(64, 2048 / L);
+    const int64_t nrb = (c->n + R - 1) / R;
+    const int64_t nch_min = std::max<int64_t>(1, (c->num_cus + nrb - 1) / nrb);
+    const int64_t C = std::min<int64_t>(C0, round_up((c->n + nch_min - 1) / nch_min, 64));
+    const int kb = (int)std::max<int64_t>(1, (C * (int64_t)c->tsize + 1023) / 1024);
     const double psz = c->prec == INFOLI_FP32 ? 4.0 : 8.0;
-    if (tiles_per_block(c, c->exec.tile_cells, C) * psz <= 0.5 * mean_deg * (2.0 + (double)c->tsize)) {
+    if (tiles_per_block(c, R, tiled_chunk(c, kb)) * psz <= 0.5 * mean_deg * (2.0 + (double)c->tsize)) {
         c->exec.gather = GATHER_TILED;
-        c->exec.lds_kb = tkb;
+        c->exec.lds_kb = kb;
         c->exec.block = 1024;
-        c->exec.lanes_per_cell = 1;
+        c->exec.lanes_per_cell = L;
+        c->exec.tile_cells = R;
         return;
     }
     if (max_block_entries_fit(c, 40)) {
@@ -1094,9 +1121,9 @@ extern "C" int infoli_set_exec(infoli_t *c, const infoli_exec *e)
         return infoli_fail(INFOLI_ERR_ARG, "infoli_set_exec: mode %d not available", x.mode);
     if (lidx(x.lanes_per_cell) < 0)
         return infoli_fail(INFOLI_ERR_ARG, "lanes_per_cell %d not in {1,2,4,8,16}", x.lanes_per_cell);
-    if (x.gather == GATHER_TILED ? (x.block != 256 && x.block != 512 && x.block != 1024) : x.block != kBlock)
-        return infoli_fail(INFOLI_ERR_ARG, "block %d: must be %d (256, 512 or 1024 for the tiled gather)",
-                           x.block, kBlock);
+    if (x.gather != GATHER_TILED) x.block = kBlock;   // the fused step kernel has one block size
+    if (x.block != 256 && x.block != 512 && x.block != 1024)
+        return infoli_fail(INFOLI_ERR_ARG, "block %d: the tiled gather takes 256, 512 or 1024", x.block);
     if (x.steps_per_launch < 1 || x.steps_per_launch > 100000)
         return infoli_fail(INFOLI_ERR_ARG, "steps_per_launch %d out of range", x.steps_per_launch);
     if (x.gather < GATHER_DIRECT || x.gather > GATHER_TILED)
@@ -1104,6 +1131,8 @@ extern "C" int infoli_set_exec(infoli_t *c, const infoli_exec *e)
     if (x.gather == GATHER_TILED && (x.tile_cells < 64 || x.tile_cells > 16384 || x.tile_cells % 64))
         return infoli_fail(INFOLI_ERR_ARG, "tile_cells %d: must be a multiple of 64 in [64, 16384]",
                            x.tile_cells);
+    if (x.fast_math && x.gather != GATHER_TILED)
+        return infoli_fail(INFOLI_ERR_ARG, "fast_math is available with the tiled gather only");
     if (x.gather == GATHER_TILED && x.mode == INFOLI_EXEC_PERSISTENT)
         return infoli_fail(INFOLI_ERR_ARG, "the tiled gather runs as two kernels per step; "
                                            "use the step or graph launch mode");
@@ -1432,16 +1461,16 @@ extern "C" int infoli_autotune(infoli_t *c, int64_t trial_steps, infoli_exec *ch
     const int spl = (int)std::min<int64_t>(trial_steps, 500);
     for (int L : Ls)
         for (int mode : modes) {
-            cands.push_back(infoli_exec{mode, L, kBlock, spl, GATHER_DIRECT, kDefaultLdsKb, 1024, 1});
+            cands.push_back(infoli_exec{mode, L, kBlock, spl, GATHER_DIRECT, kDefaultLdsKb, 1024, 1, 0});
             if ((size_t)c->n * c->tsize <= kVldsMaxBytes)
-                cands.push_back(infoli_exec{mode, L, kBlock, spl, GATHER_VLDS, kDefaultLdsKb, 1024, 1});
+                cands.push_back(infoli_exec{mode, L, kBlock, spl, GATHER_VLDS, kDefaultLdsKb, 1024, 1, 0});
             for (int kb : kbs)
                 if (max_block_entries_fit(c, kb))
-                    cands.push_back(infoli_exec{mode, L, kBlock, spl, GATHER_STAGED, kb, 1024, 1});
+                    cands.push_back(infoli_exec{mode, L, kBlock, spl, GATHER_STAGED, kb, 1024, 1, 0});
             if (mode == INFOLI_EXEC_GRAPH && L <= 2)
                 for (int kb : {32, 48})
                     for (int R : {1024, 2048})
-                        cands.push_back(infoli_exec{mode, L, 1024, spl, GATHER_TILED, kb, R, 1});
+                        cands.push_back(infoli_exec{mode, L, 1024, spl, GATHER_TILED, kb, R, 1, 0});
         }
     double best = 1e300;
     infoli_exec best_e = orig;

@@ -40,9 +40,11 @@ struct Model {
 /* One tile of the 2D-tiled gap-junction kernel: the junctions from sender
  * chunk `ch` into receiver block `rb`. */
 struct TileDesc {
-    int32_t rb, ch, nsl, pad;
+    int32_t rb, ch, nsl;
+    int32_t wconst;            // 1: every junction of the tile has conductance wc (tw not read)
     int64_t sl0;               // first slice in tsp
     int64_t rl0;               // first receiver slot in rlist / rdeg
+    double wc;
 };
 
 template <typename T>
@@ -123,28 +125,46 @@ struct CellState64 {
 __device__ __forceinline__ float fmin_ref(float a, float b) { return (a < b) ? a : b; }
 __device__ __forceinline__ double fmin_ref(double a, double b) { return (a < b) ? a : b; }
 
+/* FM = false: the reference's float arithmetic (libm-accurate expf, IEEE
+ * division).  FM = true: the hardware exp2 (x * log2 e first) and division
+ * as a multiply by the hardware reciprocal, each within ~2 ulp. */
+template <bool FM>
+__device__ __forceinline__ float E(float x)
+{
+    if constexpr (FM) return __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
+    else return expf(x);
+}
+
+template <bool FM>
+__device__ __forceinline__ float D(float a, float b)
+{
+    if constexpr (FM) return a * __builtin_amdgcn_rcpf(b);
+    else return a / b;
+}
+
 /* dendrite, OMP:561-593; updates q, r, s, ca, icah */
+template <bool FM = false>
 __device__ __forceinline__ float ode_dend(const Model<float> &P, CellState32 &c, float Vd, float Vs,
                                           float I_c, float iApp)
 {
     // Dend H current OMP:561-564
-    float q_inf = 1 / (1 + expf((Vd + 80) / 4));
-    float tau_q = 1 / (expf(-0.086f * Vd - 14.6f) + expf(0.070f * Vd - 1.87f));
-    float dq_dt = (q_inf - c.q) / tau_q;
+    float q_inf = D<FM>(1, 1 + E<FM>((Vd + 80) / 4));
+    float tau_q = D<FM>(1, E<FM>(-0.086f * Vd - 14.6f) + E<FM>(0.070f * Vd - 1.87f));
+    float dq_dt = D<FM>(q_inf - c.q, tau_q);
     c.q = P.DELTA * dq_dt + c.q;
     // Dend Ca current OMP:567-572
-    float alpha_r = 1.7f / (1 + expf(-(Vd - 5) / 13.9f));
-    float beta_r = 0.02f * (Vd + 8.5f) / (expf((Vd + 8.5f) / 5) - 1);
-    float r_inf = alpha_r / (alpha_r + beta_r);
-    float tau_r = 5 / (alpha_r + beta_r);
-    float dr_dt = (r_inf - c.r) / tau_r;
+    float alpha_r = D<FM>(1.7f, 1 + E<FM>(D<FM>(-(Vd - 5), 13.9f)));
+    float beta_r = D<FM>(0.02f * (Vd + 8.5f), E<FM>(D<FM>(Vd + 8.5f, 5)) - 1);
+    float r_inf = D<FM>(alpha_r, alpha_r + beta_r);
+    float tau_r = D<FM>(5, alpha_r + beta_r);
+    float dr_dt = D<FM>(r_inf - c.r, tau_r);
     c.r = P.DELTA * dr_dt + c.r;
     // Dend K current OMP:575-580 (previous Ca2Plus)
     float alpha_s = fmin_ref((0.00002f * c.ca), 0.01f);
     float beta_s = 0.015f;
-    float s_inf = alpha_s / (alpha_s + beta_s);
-    float tau_s = 1 / (alpha_s + beta_s);
-    float ds_dt = (s_inf - c.s) / tau_s;
+    float s_inf = D<FM>(alpha_s, alpha_s + beta_s);
+    float tau_s = D<FM>(1, alpha_s + beta_s);
+    float ds_dt = D<FM>(s_inf - c.s, tau_s);
     c.s = P.DELTA * ds_dt + c.s;
     // Dend Cal OMP:583-584 (previous I_CaH)
     float dCa_dt = -3 * c.icah - 0.075f * c.ca;
@@ -155,43 +175,44 @@ __device__ __forceinline__ float ode_dend(const Model<float> &P, CellState32 &c,
     float I_K_Ca = P.G_K_CA * c.s * (Vd - P.V_K);
     float I_ld = P.G_LD * (Vd - P.V_L);
     float I_h = P.G_H * c.q * (Vd - P.V_H);
-    float dVd_dt = (-(I_CaH + I_sd + I_ld + I_K_Ca + I_c + I_h) + iApp) / P.C_M;
+    float dVd_dt = D<FM>(-(I_CaH + I_sd + I_ld + I_K_Ca + I_c + I_h) + iApp, P.C_M);
     c.icah = I_CaH;
     return P.DELTA * dVd_dt + Vd;
 }
 
 /* soma, OMP:599-642; updates k, l, m, h, n, p, xs */
+template <bool FM = false>
 __device__ __forceinline__ float ode_soma(const Model<float> &P, CellState32 &c, float Vd, float Vs, float Va)
 {
     // Soma calcium OMP:599-606
-    float k_inf = (1 / (1 + expf(-1 * (Vs + 61) / 4.2f)));
-    float l_inf = (1 / (1 + expf((Vs + 85.5f) / 8.5f)));
-    float tau_l = ((20 * expf((Vs + 160) / 30) / (1 + expf((Vs + 84) / 7.3f))) + 35);
+    float k_inf = D<FM>(1, 1 + E<FM>(D<FM>(-1 * (Vs + 61), 4.2f)));
+    float l_inf = D<FM>(1, 1 + E<FM>(D<FM>(Vs + 85.5f, 8.5f)));
+    float tau_l = D<FM>(20 * E<FM>(D<FM>(Vs + 160, 30)), 1 + E<FM>(D<FM>(Vs + 84, 7.3f))) + 35;
     float dk_dt = (k_inf - c.k) / 1.0f;
-    float dl_dt = (l_inf - c.l) / tau_l;
+    float dl_dt = D<FM>(l_inf - c.l, tau_l);
     c.k = P.DELTA * dk_dt + c.k;
     c.l = P.DELTA * dl_dt + c.l;
     // Soma sodium OMP:609-614
-    float m_inf = 1 / (1 + (expf((-30 - Vs) / 5.5f)));
-    float h_inf = 1 / (1 + (expf((-70 - Vs) / -5.8f)));
-    float tau_h = 3 * expf((-40 - Vs) / 33);
-    float dh_dt = (h_inf - c.h) / tau_h;
+    float m_inf = D<FM>(1, 1 + E<FM>(D<FM>(-30 - Vs, 5.5f)));
+    float h_inf = D<FM>(1, 1 + E<FM>(D<FM>(-70 - Vs, -5.8f)));
+    float tau_h = 3 * E<FM>(D<FM>(-40 - Vs, 33));
+    float dh_dt = D<FM>(h_inf - c.h, tau_h);
     c.m = m_inf;
     c.h = c.h + P.DELTA * dh_dt;
     // Soma potassium OMP:617-624
-    float n_inf = 1 / (1 + expf((-3 - Vs) / 10));
-    float p_inf = 1 / (1 + expf((-51 - Vs) / -12));
-    float tau_n = 5 + (47 * expf(-(-50 - Vs) / 900));
-    float dn_dt = (n_inf - c.n) / tau_n;
-    float dp_dt = (p_inf - c.p) / tau_n;
+    float n_inf = D<FM>(1, 1 + E<FM>(D<FM>(-3 - Vs, 10)));
+    float p_inf = D<FM>(1, 1 + E<FM>(D<FM>(-51 - Vs, -12)));
+    float tau_n = 5 + (47 * E<FM>(D<FM>(-(-50 - Vs), 900)));
+    float dn_dt = D<FM>(n_inf - c.n, tau_n);
+    float dp_dt = D<FM>(p_inf - c.p, tau_n);
     c.n = P.DELTA * dn_dt + c.n;
     c.p = P.DELTA * dp_dt + c.p;
     // Soma potassium X OMP:627-632 (literal 0.05f)
-    float alpha_x_s = 0.13f * (Vs + 25) / (1 - expf(-(Vs + 25) / 10));
-    float beta_x_s = 1.69f * expf(-0.0125f * (Vs + 35));
-    float x_inf_s = alpha_x_s / (alpha_x_s + beta_x_s);
-    float tau_x_s = 1 / (alpha_x_s + beta_x_s);
-    float dx_dt_s = (x_inf_s - c.xs) / tau_x_s;
+    float alpha_x_s = D<FM>(0.13f * (Vs + 25), 1 - E<FM>(D<FM>(-(Vs + 25), 10)));
+    float beta_x_s = 1.69f * E<FM>(-0.0125f * (Vs + 35));
+    float x_inf_s = D<FM>(alpha_x_s, alpha_x_s + beta_x_s);
+    float tau_x_s = D<FM>(1, alpha_x_s + beta_x_s);
+    float dx_dt_s = D<FM>(x_inf_s - c.xs, tau_x_s);
     c.xs = 0.05f * dx_dt_s + c.xs;
     // somatic currents OMP:635-642
     float I_ds = P.G_ds * (Vs - Vd);
@@ -203,26 +224,27 @@ __device__ __forceinline__ float ode_soma(const Model<float> &P, CellState32 &c,
     float I_Kdr_s = P.G_KDR_S * (n2 * n2) * (Vs - P.V_K);
     float I_K_s = P.G_K_S * (xs2 * xs2) * (Vs - P.V_K);
     float I_as = P.G_as * (Vs - Va);
-    float dVs_dt = (-(I_CaL + I_ds + I_as + I_Na_s + I_ls + I_Kdr_s + I_K_s)) / P.C_M;
+    float dVs_dt = D<FM>(-(I_CaL + I_ds + I_as + I_Na_s + I_ls + I_Kdr_s + I_K_s), P.C_M);
     return P.DELTA * dVs_dt + Vs;
 }
 
 /* axon, OMP:648-668; updates ma, ha, xa */
+template <bool FM = false>
 __device__ __forceinline__ float ode_axon(const Model<float> &P, CellState32 &c, float Vs, float Va)
 {
     // Axon sodium OMP:648-653
-    float m_inf_a = 1 / (1 + (expf((-30 - Va) / 5.5f)));
-    float h_inf_a = 1 / (1 + (expf((-60 - Va) / -5.8f)));
-    float tau_h_a = 1.5f * expf((-40 - Va) / 33);
-    float dh_dt_a = (h_inf_a - c.ha) / tau_h_a;
+    float m_inf_a = D<FM>(1, 1 + E<FM>(D<FM>(-30 - Va, 5.5f)));
+    float h_inf_a = D<FM>(1, 1 + E<FM>(D<FM>(-60 - Va, -5.8f)));
+    float tau_h_a = 1.5f * E<FM>(D<FM>(-40 - Va, 33));
+    float dh_dt_a = D<FM>(h_inf_a - c.ha, tau_h_a);
     c.ma = m_inf_a;
     c.ha = c.ha + P.DELTA * dh_dt_a;
     // Axon potassium OMP:656-661
-    float alpha_x_a = 0.13f * (Va + 25) / (1 - expf(-(Va + 25) / 10));
-    float beta_x_a = 1.69f * expf(-0.0125f * (Va + 35));
-    float x_inf_a = alpha_x_a / (alpha_x_a + beta_x_a);
-    float tau_x_a = 1 / (alpha_x_a + beta_x_a);
-    float dx_dt_a = (x_inf_a - c.xa) / tau_x_a;
+    float alpha_x_a = D<FM>(0.13f * (Va + 25), 1 - E<FM>(D<FM>(-(Va + 25), 10)));
+    float beta_x_a = 1.69f * E<FM>(-0.0125f * (Va + 35));
+    float x_inf_a = D<FM>(alpha_x_a, alpha_x_a + beta_x_a);
+    float tau_x_a = D<FM>(1, alpha_x_a + beta_x_a);
+    float dx_dt_a = D<FM>(x_inf_a - c.xa, tau_x_a);
     c.xa = 0.05f * dx_dt_a + c.xa;
     // axonal currents OMP:664-668
     float ma3 = c.ma * c.ma * c.ma, xa2 = c.xa * c.xa;
@@ -230,7 +252,7 @@ __device__ __forceinline__ float ode_axon(const Model<float> &P, CellState32 &c,
     float I_la = P.G_LA * (Va - P.V_L);
     float I_sa = P.G_sa * (Va - Vs);
     float I_K_a = P.G_K_A * (xa2 * xa2) * (Va - P.V_K);
-    float dVa_dt = (-(I_K_a + I_sa + I_la + I_Na_a)) / P.C_M;
+    float dVa_dt = D<FM>(-(I_K_a + I_sa + I_la + I_Na_a), P.C_M);
     return P.DELTA * dVa_dt + Va;
 }
 
@@ -353,14 +375,48 @@ __device__ __forceinline__ double ode_axon(const Model<double> &P, CellState64 &
     return P.DELTA * dVa_dt + prevV_axon;
 }
 
+/* the compartments with the arithmetic switch (FP64 has one arithmetic) */
+template <bool FM>
+__device__ __forceinline__ double dend(const Model<double> &P, CellState64 &c, double Vd, double Vs, double I_c,
+                                       double iApp)
+{
+    return ode_dend(P, c, Vd, Vs, I_c, iApp);
+}
+template <bool FM>
+__device__ __forceinline__ float dend(const Model<float> &P, CellState32 &c, float Vd, float Vs, float I_c,
+                                      float iApp)
+{
+    return ode_dend<FM>(P, c, Vd, Vs, I_c, iApp);
+}
+template <bool FM>
+__device__ __forceinline__ double soma(const Model<double> &P, CellState64 &c, double Vd, double Vs, double Va)
+{
+    return ode_soma(P, c, Vd, Vs, Va);
+}
+template <bool FM>
+__device__ __forceinline__ float soma(const Model<float> &P, CellState32 &c, float Vd, float Vs, float Va)
+{
+    return ode_soma<FM>(P, c, Vd, Vs, Va);
+}
+template <bool FM>
+__device__ __forceinline__ double axon(const Model<double> &P, CellState64 &c, double Vs, double Va)
+{
+    return ode_axon(P, c, Vs, Va);
+}
+template <bool FM>
+__device__ __forceinline__ float axon(const Model<float> &P, CellState32 &c, float Vs, float Va)
+{
+    return ode_axon<FM>(P, c, Vs, Va);
+}
+
 /* all three compartments, voltages committed last */
-template <typename T, typename C>
+template <bool FM = false, typename T, typename C>
 __device__ __forceinline__ void ode(const Model<T> &P, C &c, T I_c, T iApp)
 {
     const T Vd = c.vd, Vs = c.vs, Va = c.va;
-    const T vd = ode_dend(P, c, Vd, Vs, I_c, iApp);
-    const T vs = ode_soma(P, c, Vd, Vs, Va);
-    const T va = ode_axon(P, c, Vs, Va);
+    const T vd = dend<FM>(P, c, Vd, Vs, I_c, iApp);
+    const T vs = soma<FM>(P, c, Vd, Vs, Va);
+    const T va = axon<FM>(P, c, Vs, Va);
     c.vd = vd;
     c.vs = vs;
     c.va = va;
@@ -528,6 +584,30 @@ __device__ __forceinline__ void row_terms(Acc<T, KAHAN> &acc, T vi, int d, int l
         for (int u = 0; u < kUnroll; u++) {
             if (kk0 + u < W) {
                 const T term = gj_term(vi, vv[u], wv[u]);
+                acc.add(((kk0 + u) * L + l < d) ? term : T(0));
+            }
+        }
+    }
+}
+
+/* sender index only; every junction has conductance wc */
+template <typename T, bool KAHAN, int L, typename IdxT, typename VOf>
+__device__ __forceinline__ void row_terms_idx_cw(Acc<T, KAHAN> &acc, T vi, int d, int l, int W, int64_t base,
+                                                 T wc, const IdxT *__restrict__ idx, VOf v_of)
+{
+    for (int kk0 = 0; kk0 < W; kk0 += kUnroll) {
+        int jv[kUnroll];
+        T vv[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++)
+            if (kk0 + u < W) jv[u] = (int)idx[base + (int64_t)(kk0 + u) * 64];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++)
+            if (kk0 + u < W) vv[u] = v_of(jv[u]);
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            if (kk0 + u < W) {
+                const T term = gj_term(vi, vv[u], wc);
                 acc.add(((kk0 + u) * L + l < d) ? term : T(0));
             }
         }
@@ -840,8 +920,12 @@ __global__ __launch_bounds__(NT) void k_tile_gj(StepArgs<T> a, int64_t k)
             const T vi = vin[cell];
             const int64_t base0 = a.tsp[td.sl0 + sl];
             const int W = (int)((a.tsp[td.sl0 + sl + 1] - base0) >> 6);
-            row_terms_idx<T, KAHAN, L>(acc, vi, d, l, W, base0 + cw * L + l, a.tw, a.tcol,
-                                       [&](int j) { return vch[j]; });
+            if (td.wconst)
+                row_terms_idx_cw<T, KAHAN, L>(acc, vi, d, l, W, base0 + cw * L + l, (T)td.wc, a.tcol,
+                                              [&](int j) { return vch[j]; });
+            else
+                row_terms_idx<T, KAHAN, L>(acc, vi, d, l, W, base0 + cw * L + l, a.tw, a.tcol,
+                                           [&](int j) { return vch[j]; });
         }
         if constexpr (L > 1) {
 #pragma unroll
@@ -883,7 +967,7 @@ __device__ __forceinline__ T tiled_ic(const StepArgs<T> &a, int64_t cell)
 }
 
 /* compartment update of every cell, I_c from the tile partials */
-template <typename T, bool KAHAN>
+template <typename T, bool KAHAN, bool FM>
 __global__ __launch_bounds__(kBlock) void k_tile_ode(StepArgs<T> a, int64_t k)
 {
     const int64_t t = (a.d_step ? *a.d_step : 0) + k;
@@ -891,14 +975,14 @@ __global__ __launch_bounds__(kBlock) void k_tile_ode(StepArgs<T> a, int64_t k)
     if (cell >= a.n) return;
     typename CellOf<T>::type c;
     load_cell(a, cell, t, c);
-    ode(a.m, c, tiled_ic<T, KAHAN>(a, cell), iapp_of(a, cell, t));
+    ode<FM>(a.m, c, tiled_ic<T, KAHAN>(a, cell), iapp_of(a, cell, t));
     store_cell(a, cell, t, c);
 }
 
 /* The same update with one wavefront per compartment: a workgroup of 192
  * threads covers 64 cells; wave 0 runs the dendrites, wave 1 the somata,
  * wave 2 the axons, each from the start-of-step voltages. */
-template <typename T, bool KAHAN>
+template <typename T, bool KAHAN, bool FM>
 __global__ __launch_bounds__(192) void k_tile_ode3(StepArgs<T> a, int64_t k)
 {
     const int64_t t = (a.d_step ? *a.d_step : 0) + k;
@@ -916,7 +1000,7 @@ __global__ __launch_bounds__(192) void k_tile_ode3(StepArgs<T> a, int64_t k)
         c.s = st[POTASSIUM_S * np + cell];
         c.ca = st[CA2PLUS * np + cell];
         c.icah = st[I_CAH * np + cell];
-        const T vd = ode_dend(a.m, c, Vd, Vs, tiled_ic<T, KAHAN>(a, cell), iapp_of(a, cell, t));
+        const T vd = dend<FM>(a.m, c, Vd, Vs, tiled_ic<T, KAHAN>(a, cell), iapp_of(a, cell, t));
         st[HCURRENT_Q * np + cell] = c.q;
         st[CALCIUM_R * np + cell] = c.r;
         st[POTASSIUM_S * np + cell] = c.s;
@@ -932,7 +1016,7 @@ __global__ __launch_bounds__(192) void k_tile_ode3(StepArgs<T> a, int64_t k)
         c.p = st[POTASSIUM_P * np + cell];
         c.xs = st[POTASSIUM_X_S * np + cell];
         c.gcal = st[G_CAL * np + cell];
-        const T vs = ode_soma(a.m, c, Vd, Vs, Va);
+        const T vs = soma<FM>(a.m, c, Vd, Vs, Va);
         st[CALCIUM_K * np + cell] = c.k;
         st[CALCIUM_L * np + cell] = c.l;
         st[SODIUM_M * np + cell] = c.m;
@@ -945,7 +1029,7 @@ __global__ __launch_bounds__(192) void k_tile_ode3(StepArgs<T> a, int64_t k)
         const T Va = a.va[cur][cell];
         c.ha = st[SODIUM_H_A * np + cell];
         c.xa = st[POTASSIUM_X_A * np + cell];
-        const T va = ode_axon(a.m, c, Vs, Va);
+        const T va = axon<FM>(a.m, c, Vs, Va);
         st[SODIUM_M_A * np + cell] = c.ma;
         st[SODIUM_H_A * np + cell] = c.ha;
         st[POTASSIUM_X_A * np + cell] = c.xa;

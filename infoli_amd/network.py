@@ -43,7 +43,8 @@ class _Exec(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("lanes_per_cell", ctypes.c_int),
                 ("block", ctypes.c_int), ("steps_per_launch", ctypes.c_int),
                 ("gather", ctypes.c_int), ("lds_kb", ctypes.c_int),
-                ("tile_cells", ctypes.c_int), ("ode_split", ctypes.c_int)]
+                ("tile_cells", ctypes.c_int), ("ode_split", ctypes.c_int),
+                ("fast_math", ctypes.c_int)]
 
 
 _lib = None
@@ -290,14 +291,15 @@ class Network:
         _check(library().infoli_get_exec(self._h, ctypes.byref(e)), "get_exec")
         return dict(mode=e.mode, lanes_per_cell=e.lanes_per_cell, block=e.block,
                     steps_per_launch=e.steps_per_launch, gather=e.gather, lds_kb=e.lds_kb,
-                    tile_cells=e.tile_cells, ode_split=e.ode_split)
+                    tile_cells=e.tile_cells, ode_split=e.ode_split, fast_math=e.fast_math)
 
     def set_exec(self, mode=None, lanes_per_cell=None, block=None, steps_per_launch=None,
-                 gather=None, lds_kb=None, tile_cells=None, ode_split=None):
+                 gather=None, lds_kb=None, tile_cells=None, ode_split=None, fast_math=None):
         cur = self.exec
         for k, v in (("mode", mode), ("lanes_per_cell", lanes_per_cell), ("block", block),
                      ("steps_per_launch", steps_per_launch), ("gather", gather),
-                     ("lds_kb", lds_kb), ("tile_cells", tile_cells), ("ode_split", ode_split)):
+                     ("lds_kb", lds_kb), ("tile_cells", tile_cells), ("ode_split", ode_split),
+                     ("fast_math", fast_math)):
             if v is not None:
                 cur[k] = int(v)
         e = _Exec(**cur)

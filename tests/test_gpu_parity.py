@@ -332,6 +332,46 @@ def test_sparse_100k_full_size_precision():
     assert err <= 1e-3
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp32k"])
+def test_fast_math_compartments(prec):
+    """Hardware exp2 / reciprocal in the compartment update (tiled path):
+    still within the FP32 tolerance of the FP64 oracle."""
+    say(f"fast math {prec}")
+    n, steps = 700, 2000
+    rp, col, w, gcal, iapp = _random_case(n)
+    mask = I.gen_mask(3, 4, n, 0.3)
+    net = make_net(n, prec, rp, col, w, gcal, iapp)
+    net.set_iapp_pulse(6.0, 500, 900, mask)
+    net.set_exec(gather=I.GATHER_TILED, block=1024, lds_kb=1, tile_cells=256, lanes_per_cell=1,
+                 fast_math=1)
+    out = net.advance(steps, trace_stride=5)
+    net.close()
+    ref64, _ = oracle_ref("fp64", n, steps, rp, col, w, gcal, iapp, mask=mask, amp=6.0, t0=500,
+                          t1=900, stride=5)
+    assert np.abs(out - ref64).max() < 1e-3
+
+
+def test_sparse_100k_fast_math_precision():
+    """FP32 + compensated sums with the fast compartment arithmetic vs FP64
+    over the first 100 ms of the 100k-cell sparse network."""
+    say("sparse_100k fast math")
+    wl = W.sparse_100k()
+    nets = {}
+    for prec in ("fp64", "fp32k"):
+        net = I.Network(wl["n"], precision=prec)
+        W.load_into(net, wl)
+        nets[prec] = net
+    nets["fp32k"].set_exec(fast_math=1)
+    assert nets["fp32k"].exec["gather"] == I.GATHER_TILED
+    t64 = nets["fp64"].advance(2000, trace_stride=1)
+    t32 = nets["fp32k"].advance(2000, trace_stride=1)
+    err = np.abs(t64 - t32).max()
+    say(f"  max |dV| FP32+compensated (fast compartments) vs FP64 over 100 ms: {err:.3e} mV")
+    for net in nets.values():
+        net.close()
+    assert err <= 1e-3
+
+
 def test_driver_reproduces_regression_baseline(golden):
     say("driver regression")
     exe = os.path.join(O.ROOT, "infoli_amd", "bin", "infoli.x")
