@@ -35,6 +35,35 @@ __global__ void copy_k(const float4 *__restrict__ a, float4 *__restrict__ b, lon
     for (; i < n; i += s) b[i] = a[i];
 }
 
+/* read-only stream: 4 independent 16-byte loads in flight per thread */
+__global__ void read_k(const float4 *__restrict__ a, float *out, long n)
+{
+    const long s = (long)gridDim.x * blockDim.x;
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    float acc = 0.f;
+    for (; i + 3 * s < n; i += 4 * s) {
+        const float4 x0 = a[i], x1 = a[i + s], x2 = a[i + 2 * s], x3 = a[i + 3 * s];
+        acc += (x0.x + x1.y) + (x2.z + x3.w);
+    }
+    for (; i < n; i += s) acc += a[i].x;
+    out[(long)blockIdx.x * blockDim.x + threadIdx.x] = acc;
+}
+
+/* copy with 4 independent 16-byte loads in flight per thread */
+__global__ void copy4_k(const float4 *__restrict__ a, float4 *__restrict__ b, long n)
+{
+    const long s = (long)gridDim.x * blockDim.x;
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * s < n; i += 4 * s) {
+        const float4 x0 = a[i], x1 = a[i + s], x2 = a[i + 2 * s], x3 = a[i + 3 * s];
+        b[i] = x0;
+        b[i + s] = x1;
+        b[i + 2 * s] = x2;
+        b[i + 3 * s] = x3;
+    }
+    for (; i < n; i += s) b[i] = a[i];
+}
+
 __global__ void gather_global(const int *__restrict__ idx, const float *__restrict__ v, float *out,
                               int per, long nthreads)
 {
@@ -97,6 +126,25 @@ int main(int argc, char **argv)
             const float ms = time_ms(e0, e1);
             printf("copy %ld MiB: %.3f ms, %.1f GB/s read+write\n", mib, ms, 2.0 * n4 * 16 / ms / 1e6);
         }
+        float *red;
+        CHK(hipMalloc(&red, 256L * 16 * 256 * sizeof(float)));
+        for (int rep = 0; rep < 3; rep++) {
+            CHK(hipEventRecord(e0));
+            hipLaunchKernelGGL(copy4_k, dim3(256 * 16), dim3(256), 0, 0, a, b, n4);
+            CHK(hipGetLastError());
+            CHK(hipEventRecord(e1));
+            const float ms = time_ms(e0, e1);
+            printf("copy4 %ld MiB: %.3f ms, %.1f GB/s read+write\n", mib, ms, 2.0 * n4 * 16 / ms / 1e6);
+        }
+        for (int rep = 0; rep < 3; rep++) {
+            CHK(hipEventRecord(e0));
+            hipLaunchKernelGGL(read_k, dim3(256 * 16), dim3(256), 0, 0, a, red, n4);
+            CHK(hipGetLastError());
+            CHK(hipEventRecord(e1));
+            const float ms = time_ms(e0, e1);
+            printf("read %ld MiB: %.3f ms, %.1f GB/s read\n", mib, ms, 1.0 * n4 * 16 / ms / 1e6);
+        }
+        CHK(hipFree(red));
         CHK(hipFree(a));
         CHK(hipFree(b));
     } else if (!strcmp(what, "gather")) {

@@ -220,6 +220,7 @@ StepArgs<T> make_args(const infoli_ctx *c, int64_t stride)
         a.tcol = c->d_tcol;
         a.tw = (const T *)c->d_tw;
         a.part = c->d_part;
+        a.ntiles = (int32_t)c->tl_ntiles;
         a.rcells = c->tiled_R;
         a.ccells = (int32_t)c->tl_C;
     }
@@ -288,6 +289,10 @@ const void *const k_tile_ode3_tab[2][3] = {
     {(const void *)k_tile_ode3<double, false, false>, (const void *)k_tile_ode3<float, false, true>,
      (const void *)k_tile_ode3<float, true, true>},
 };
+const void *const k_tile_dend_tab[3] = {
+    (const void *)k_tile_dend<double, false>, (const void *)k_tile_dend<float, false>,
+    (const void *)k_tile_dend<float, true>,
+};
 const void *const k_gap_tab[3][5] = {
     INFOLI_L_TABLE(k_gap_currents, double, false),
     INFOLI_L_TABLE(k_gap_currents, float, false),
@@ -328,6 +333,17 @@ int launch_step(infoli_ctx *c, const void *args, int64_t k)
         void *kargs[] = {const_cast<void *>(args), &kk};
         const size_t lds = (size_t)c->tl_C * c->tsize;
         allow_dyn_lds(fg, lds);
+        if (c->exec.ode_split == 2) {
+            // tiles, then workgroups of block/2 somata and axons in the same
+            // launch; the dendrites (which need the tiles' partials) after
+            const int64_t per = c->exec.block / 2;
+            const int64_t nsa = (c->n + per - 1) / per;
+            HIPCHK(hipLaunchKernel(fg, dim3((unsigned)(c->tl_ntiles + nsa)), dim3(c->exec.block), kargs, lds,
+                                   c->stream));
+            HIPCHK(hipLaunchKernel(k_tile_dend_tab[c->prec], dim3((unsigned)(c->npad / kBlock)), dim3(kBlock), kargs,
+                                   0, c->stream));
+            return INFOLI_OK;
+        }
         HIPCHK(hipLaunchKernel(fg, dim3((unsigned)c->tl_ntiles), dim3(c->exec.block), kargs, lds, c->stream));
         if (c->exec.ode_split)
             HIPCHK(hipLaunchKernel(k_tile_ode3_tab[fm][c->prec], dim3((unsigned)(c->npad / 64)), dim3(192), kargs, 0,
@@ -749,27 +765,48 @@ void default_exec(infoli_ctx *c)
     c->exec.gather = GATHER_DIRECT;
     if (mean_deg < 4) return;
     // tiled: lanes per receiver from the mean in-tile degree of a first
-    // layout guess, receivers per tile inversely, then sender chunks small
-    // enough for at least one tile per CU
+    // layout guess (about 16 junctions per lane), receivers per tile
+    // inversely, sender chunks small enough for one tile per CU (two when
+    // the in-tile degree is high, so partial sums stay cheap), then the
+    // lanes again from the in-tile degree of that layout
     const int64_t C0 = tiled_chunk(c, 48);
     const int R0 = 2048;
-    const int64_t nrb0 = (c->n + R0 - 1) / R0;
-    const double tiles0 = tiles_per_block(c, R0, C0) * (double)nrb0;
-    const double dc = tiles0 > 0 ? (double)c->col.size() / (tiles0 * R0) : 0.0;
-    int L = 1;
-    while (L < 16 && 2.0 * L * 16.0 <= dc) L *= 2;
-    const int R = std::max(64, 2048 / L);
+    const auto in_tile_degree = [c](int R, int64_t C) {
+        const double tiles = tiles_per_block(c, R, C) * (double)((c->n + R - 1) / R);
+        return tiles > 0 ? (double)c->col.size() / (tiles * R) : 0.0;
+    };
+    const auto lanes_for = [](double dc) {
+        int L = 1;
+        while (L < 16 && 32.0 * L <= dc) L *= 2;
+        return L;
+    };
+    const int L0 = lanes_for(in_tile_degree(R0, C0));
+    const int R = std::max(64, 2048 / L0);
     const int64_t nrb = (c->n + R - 1) / R;
-    const int64_t nch_min = std::max<int64_t>(1, (c->num_cus + nrb - 1) / nrb);
+    const int64_t want = (L0 >= 4 ? 2 : 1) * (int64_t)c->num_cus;
+    const int64_t nch_min = std::max<int64_t>(1, (want + nrb - 1) / nrb);
     const int64_t C = std::min<int64_t>(C0, round_up((c->n + nch_min - 1) / nch_min, 64));
     const int kb = (int)std::max<int64_t>(1, (C * (int64_t)c->tsize + 1023) / 1024);
+    const int64_t Ck = tiled_chunk(c, kb);
+    const int L = lanes_for(in_tile_degree(R, Ck));
     const double psz = c->prec == INFOLI_FP32 ? 4.0 : 8.0;
-    if (tiles_per_block(c, R, tiled_chunk(c, kb)) * psz <= 0.5 * mean_deg * (2.0 + (double)c->tsize)) {
+    const double tpb = tiles_per_block(c, R, Ck);
+    if (tpb * psz <= 0.5 * mean_deg * (2.0 + (double)c->tsize)) {
         c->exec.gather = GATHER_TILED;
         c->exec.lds_kb = kb;
         c->exec.block = 1024;
         c->exec.lanes_per_cell = L;
         c->exec.tile_cells = R;
+        // somata and axons beside the tiles when the tiles need more than
+        // one round of workgroup slots (they then fill the last round);
+        // with a single round they would only lengthen it
+        const void *fg = k_tile_gj_tab[c->prec][ntidx(1024)][lidx(L)];
+        const size_t lds = (size_t)Ck * c->tsize;
+        allow_dyn_lds(fg, lds);
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fg, 1024, lds) != hipSuccess) per_cu = 1;
+        const double ntiles = tpb * (double)((c->n + R - 1) / R);
+        c->exec.ode_split = ntiles > (double)std::max(per_cu, 1) * c->num_cus ? 2 : 1;
         return;
     }
     if (max_block_entries_fit(c, 40)) {
@@ -1133,6 +1170,10 @@ extern "C" int infoli_set_exec(infoli_t *c, const infoli_exec *e)
                            x.tile_cells);
     if (x.fast_math && x.gather != GATHER_TILED)
         return infoli_fail(INFOLI_ERR_ARG, "fast_math is available with the tiled gather only");
+    if (x.ode_split < 0 || x.ode_split > 2)
+        return infoli_fail(INFOLI_ERR_ARG, "ode_split %d not in {0, 1, 2}", x.ode_split);
+    if (x.fast_math && x.ode_split == 2)
+        return infoli_fail(INFOLI_ERR_ARG, "fast_math is available with ode_split 0 or 1 only");
     if (x.gather == GATHER_TILED && x.mode == INFOLI_EXEC_PERSISTENT)
         return infoli_fail(INFOLI_ERR_ARG, "the tiled gather runs as two kernels per step; "
                                            "use the step or graph launch mode");

@@ -87,6 +87,7 @@ struct StepArgs {
     const uint16_t *tcol;      // sender index local to the chunk
     const T *tw;               // conductance
     void *part;                // partial sums, [tile][receiver block size] (sum, comp) pairs
+    int32_t ntiles;            // tiles (workgroups past them run somata and axons)
     int32_t rcells;            // receivers per block (multiple of 64)
     int32_t ccells;            // senders per chunk
     Model<T> m;
@@ -590,31 +591,7 @@ __device__ __forceinline__ void row_terms(Acc<T, KAHAN> &acc, T vi, int d, int l
     }
 }
 
-/* sender index only; every junction has conductance wc */
-template <typename T, bool KAHAN, int L, typename IdxT, typename VOf>
-__device__ __forceinline__ void row_terms_idx_cw(Acc<T, KAHAN> &acc, T vi, int d, int l, int W, int64_t base,
-                                                 T wc, const IdxT *__restrict__ idx, VOf v_of)
-{
-    for (int kk0 = 0; kk0 < W; kk0 += kUnroll) {
-        int jv[kUnroll];
-        T vv[kUnroll];
-#pragma unroll
-        for (int u = 0; u < kUnroll; u++)
-            if (kk0 + u < W) jv[u] = (int)idx[base + (int64_t)(kk0 + u) * 64];
-#pragma unroll
-        for (int u = 0; u < kUnroll; u++)
-            if (kk0 + u < W) vv[u] = v_of(jv[u]);
-#pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-            if (kk0 + u < W) {
-                const T term = gj_term(vi, vv[u], wc);
-                acc.add(((kk0 + u) * L + l < d) ? term : T(0));
-            }
-        }
-    }
-}
-
-/* the same with the sender index loaded first (direct and tiled gathers) */
+/* the same with the sender index loaded first (direct gather) */
 template <typename T, bool KAHAN, int L, typename IdxT, typename VOf>
 __device__ __forceinline__ void row_terms_idx(Acc<T, KAHAN> &acc, T vi, int d, int l, int W, int64_t base,
                                               const T *__restrict__ w, const IdxT *__restrict__ idx, VOf v_of)
@@ -785,16 +762,19 @@ constexpr unsigned kMaxSpins = 1u << 22;
 __device__ __forceinline__ bool grid_barrier(GridBarrier *gb, unsigned target)
 {
     __shared__ int s_ok;
-    __threadfence();
+    // every wave publishes its own stores; one arrival per workgroup; the
+    // wait spins on relaxed loads (an acquire load per spin would invalidate
+    // the L2 on every iteration) and acquires once, for the whole workgroup
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
     if (threadIdx.x == 0) {
         int ok = 1;
         if (__hip_atomic_load(&gb->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
             ok = 0;
         } else {
-            __hip_atomic_fetch_add(&gb->count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&gb->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             unsigned spins = 0;
-            while (__hip_atomic_load(&gb->count, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            while (__hip_atomic_load(&gb->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
                 if (__hip_atomic_load(&gb->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
                     ok = 0;
                     break;
@@ -806,13 +786,12 @@ __device__ __forceinline__ bool grid_barrier(GridBarrier *gb, unsigned target)
                 }
                 __builtin_amdgcn_s_sleep(2);
             }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
         s_ok = ok;
     }
     __syncthreads();
-    const int ok = s_ok;
-    __threadfence();
-    return ok != 0;
+    return s_ok != 0;
 }
 
 /* `nsteps` steps starting at *d_step; workgroups loop over cell blocks. */
@@ -873,6 +852,85 @@ __global__ __launch_bounds__(kBlock) void k_gap_currents(StepArgs<T> a, int64_t 
 template <typename T, bool KAHAN> struct PartOf { using type = T; };
 template <> struct PartOf<float, true> { using type = float2; };
 
+template <typename T> struct Vec16;
+template <> struct Vec16<float> { using type = float4; };
+template <> struct Vec16<double> { using type = double2; };
+
+/* V_dend[s0 .. s0+cn) into LDS: 16-byte loads, four per thread in flight
+ * before their LDS stores (the source is 16-byte aligned: chunks start at
+ * multiples of 64 cells of a 256-byte aligned array).  Loads and stores
+ * take an always valid index, so nothing branches: a lane past the end
+ * rewrites one of the last 64 vectors with its own value (64 distinct
+ * addresses per wavefront, so the rewrites do not pile onto one bank). */
+template <typename T, int NT>
+__device__ __forceinline__ void tile_chunk_to_lds(const T *__restrict__ src, T *__restrict__ dst, int64_t cn,
+                                                  int tid)
+{
+    using V = typename Vec16<T>::type;
+    constexpr int E = 16 / (int)sizeof(T), Q = 4;
+    const int nv = (int)(cn / E);
+    const V *__restrict__ sv = reinterpret_cast<const V *>(src);
+    V *__restrict__ dv = reinterpret_cast<V *>(dst);
+#pragma unroll 1
+    for (int b = 0; b < nv; b += Q * NT) {
+        V x[Q];
+        int j[Q];
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int i = b + q * NT + tid;
+            j[q] = i < nv ? i : max(nv - 1 - (i & 63), 0);
+            x[q] = sv[j[q]];
+        }
+#pragma unroll
+        for (int q = 0; q < Q; q++) dv[j[q]] = x[q];
+    }
+    for (int64_t i = (int64_t)nv * E + tid; i < cn; i += NT) dst[i] = src[i];
+}
+
+/* load from a wavefront-uniform pointer plus a per-lane 32-bit byte offset
+ * (the scalar-base + vector-offset form of the global load) */
+template <typename X>
+__device__ __forceinline__ X ld_off(const X *__restrict__ p, uint32_t byte_off)
+{
+    return *reinterpret_cast<const X *>(reinterpret_cast<const char *>(p) + byte_off);
+}
+
+/* One lane's share of a tile row: entries at irow/wrow[(kk*64) + loff],
+ * irow and wrow uniform across the wavefront, W the slice width.  Same
+ * arithmetic and order as row_terms_idx; WC: every junction has
+ * conductance wc. */
+template <typename T, bool KAHAN, int L, bool WC>
+__device__ __forceinline__ void tile_row_terms(Acc<T, KAHAN> &acc, T vi, int d, int l, int W, int loff,
+                                               const uint16_t *__restrict__ irow, const T *__restrict__ wrow,
+                                               T wc, const T *__restrict__ vch)
+{
+    const uint32_t oi = (uint32_t)loff * (uint32_t)sizeof(uint16_t);
+    const uint32_t ow = (uint32_t)loff * (uint32_t)sizeof(T);
+    for (int kk0 = 0; kk0 < W; kk0 += kUnroll) {
+        const uint16_t *ib = irow + (int64_t)kk0 * 64;
+        const T *wb = wrow + (int64_t)kk0 * 64;
+        int jv[kUnroll];
+        T wv[kUnroll], vv[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            if (kk0 + u < W) {
+                jv[u] = (int)ld_off(ib + u * 64, oi);
+                if constexpr (!WC) wv[u] = ld_off(wb + u * 64, ow);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++)
+            if (kk0 + u < W) vv[u] = vch[jv[u]];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            if (kk0 + u < W) {
+                const T term = gj_term(vi, vv[u], WC ? wc : wv[u]);
+                acc.add(((kk0 + u) * L + l < d) ? term : T(0));
+            }
+        }
+    }
+}
+
 template <typename T, bool KAHAN>
 __device__ __forceinline__ void part_store(void *p, int64_t i, const Acc<T, KAHAN> &a)
 {
@@ -888,54 +946,6 @@ __device__ __forceinline__ void part_merge(const void *p, int64_t i, Acc<T, KAHA
         a.merge(v.x, v.y);
     } else {
         a.merge(reinterpret_cast<const T *>(p)[i], T(0));
-    }
-}
-
-/* One tile: copy the sender chunk's V_dend to LDS, then each receiver's
- * junctions from that chunk (L lanes per receiver, list order) into a
- * partial (sum, compensation) pair. */
-template <typename T, int L, bool KAHAN, int NT>
-__global__ __launch_bounds__(NT) void k_tile_gj(StepArgs<T> a, int64_t k)
-{
-    extern __shared__ __align__(16) unsigned char dyn_lds[];
-    T *vch = reinterpret_cast<T *>(dyn_lds);
-    constexpr int CPW = 64 / L;
-    const int64_t t = (a.d_step ? *a.d_step : 0) + k;
-    const T *__restrict__ vin = a.vd[t & 1];
-    const TileDesc td = a.tiles[blockIdx.x];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int cw = lane / L, l = lane % L;
-    const int64_t s0 = (int64_t)td.ch * a.ccells;
-    const int64_t cn = (a.n - s0 < a.ccells) ? a.n - s0 : a.ccells;
-    for (int64_t i = tid; i < cn; i += NT) vch[i] = vin[s0 + i];
-    __syncthreads();
-    const int64_t cellb = (int64_t)td.rb * a.rcells;
-    for (int sl = wave; sl < td.nsl; sl += NT / 64) {
-        const int64_t slot = td.rl0 + (int64_t)sl * CPW + cw;
-        const int rloc = a.rlist[slot];
-        const int d = a.rdeg[slot];
-        const int64_t cell = cellb + rloc;
-        Acc<T, KAHAN> acc;
-        if (d > 0) {
-            const T vi = vin[cell];
-            const int64_t base0 = a.tsp[td.sl0 + sl];
-            const int W = (int)((a.tsp[td.sl0 + sl + 1] - base0) >> 6);
-            if (td.wconst)
-                row_terms_idx_cw<T, KAHAN, L>(acc, vi, d, l, W, base0 + cw * L + l, (T)td.wc, a.tcol,
-                                              [&](int j) { return vch[j]; });
-            else
-                row_terms_idx<T, KAHAN, L>(acc, vi, d, l, W, base0 + cw * L + l, a.tw, a.tcol,
-                                           [&](int j) { return vch[j]; });
-        }
-        if constexpr (L > 1) {
-#pragma unroll
-            for (int off = L / 2; off > 0; off >>= 1) {
-                T os = shfl_xor(acc.s, off);
-                T oc = KAHAN ? shfl_xor(acc.c, off) : T(0);
-                acc.merge(os, oc);
-            }
-        }
-        if (l == 0) part_store<T, KAHAN>(a.part, (int64_t)blockIdx.x * a.rcells + rloc, acc);
     }
 }
 
@@ -979,6 +989,74 @@ __global__ __launch_bounds__(kBlock) void k_tile_ode(StepArgs<T> a, int64_t k)
     store_cell(a, cell, t, c);
 }
 
+/* One compartment of one cell for step t, from the start-of-step voltages
+ * (ping-pong buffers): the dendrite needs this step's gap-junction current
+ * (merged from the tile partials), the soma and the axon do not. */
+template <typename T, bool KAHAN, bool FM>
+__device__ __forceinline__ void tile_dend_cell(const StepArgs<T> &a, int64_t cell, int64_t t)
+{
+    const int64_t np = a.npad;
+    T *st = a.st;
+    const int cur = (int)(t & 1), nxt = cur ^ 1;
+    typename CellOf<T>::type c;
+    c.q = st[HCURRENT_Q * np + cell];
+    c.r = st[CALCIUM_R * np + cell];
+    c.s = st[POTASSIUM_S * np + cell];
+    c.ca = st[CA2PLUS * np + cell];
+    c.icah = st[I_CAH * np + cell];
+    const T vd = dend<FM>(a.m, c, a.vd[cur][cell], a.vs[cur][cell], tiled_ic<T, KAHAN>(a, cell),
+                          iapp_of(a, cell, t));
+    st[HCURRENT_Q * np + cell] = c.q;
+    st[CALCIUM_R * np + cell] = c.r;
+    st[POTASSIUM_S * np + cell] = c.s;
+    st[CA2PLUS * np + cell] = c.ca;
+    st[I_CAH * np + cell] = c.icah;
+    a.vd[nxt][cell] = vd;
+}
+
+template <typename T, bool FM>
+__device__ __forceinline__ void tile_soma_cell(const StepArgs<T> &a, int64_t cell, int64_t t)
+{
+    const int64_t np = a.npad;
+    T *st = a.st;
+    const int cur = (int)(t & 1), nxt = cur ^ 1;
+    typename CellOf<T>::type c;
+    c.k = st[CALCIUM_K * np + cell];
+    c.l = st[CALCIUM_L * np + cell];
+    c.h = st[SODIUM_H * np + cell];
+    c.n = st[POTASSIUM_N * np + cell];
+    c.p = st[POTASSIUM_P * np + cell];
+    c.xs = st[POTASSIUM_X_S * np + cell];
+    c.gcal = st[G_CAL * np + cell];
+    const T vs = soma<FM>(a.m, c, a.vd[cur][cell], a.vs[cur][cell], a.va[cur][cell]);
+    st[CALCIUM_K * np + cell] = c.k;
+    st[CALCIUM_L * np + cell] = c.l;
+    st[SODIUM_M * np + cell] = c.m;
+    st[SODIUM_H * np + cell] = c.h;
+    st[POTASSIUM_N * np + cell] = c.n;
+    st[POTASSIUM_P * np + cell] = c.p;
+    st[POTASSIUM_X_S * np + cell] = c.xs;
+    a.vs[nxt][cell] = vs;
+}
+
+template <typename T, bool FM>
+__device__ __forceinline__ void tile_axon_cell(const StepArgs<T> &a, int64_t cell, int64_t t)
+{
+    const int64_t np = a.npad;
+    T *st = a.st;
+    const int cur = (int)(t & 1), nxt = cur ^ 1;
+    typename CellOf<T>::type c;
+    c.ha = st[SODIUM_H_A * np + cell];
+    c.xa = st[POTASSIUM_X_A * np + cell];
+    const T va = axon<FM>(a.m, c, a.vs[cur][cell], a.va[cur][cell]);
+    st[SODIUM_M_A * np + cell] = c.ma;
+    st[SODIUM_H_A * np + cell] = c.ha;
+    st[POTASSIUM_X_A * np + cell] = c.xa;
+    a.va[nxt][cell] = va;
+    if (a.trace && (t % a.stride) == 0)
+        a.trace[((t / a.stride) % a.trace_cap) * a.n + cell] = va;
+}
+
 /* The same update with one wavefront per compartment: a workgroup of 192
  * threads covers 64 cells; wave 0 runs the dendrites, wave 1 the somata,
  * wave 2 the axons, each from the start-of-step voltages. */
@@ -989,53 +1067,101 @@ __global__ __launch_bounds__(192) void k_tile_ode3(StepArgs<T> a, int64_t k)
     const int wave = threadIdx.x >> 6;
     const int64_t cell = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
     if (cell >= a.n) return;
-    const int64_t np = a.npad;
-    T *st = a.st;
-    const int cur = (int)(t & 1), nxt = cur ^ 1;
-    const T Vd = a.vd[cur][cell], Vs = a.vs[cur][cell];
-    typename CellOf<T>::type c;
-    if (wave == 0) {
-        c.q = st[HCURRENT_Q * np + cell];
-        c.r = st[CALCIUM_R * np + cell];
-        c.s = st[POTASSIUM_S * np + cell];
-        c.ca = st[CA2PLUS * np + cell];
-        c.icah = st[I_CAH * np + cell];
-        const T vd = dend<FM>(a.m, c, Vd, Vs, tiled_ic<T, KAHAN>(a, cell), iapp_of(a, cell, t));
-        st[HCURRENT_Q * np + cell] = c.q;
-        st[CALCIUM_R * np + cell] = c.r;
-        st[POTASSIUM_S * np + cell] = c.s;
-        st[CA2PLUS * np + cell] = c.ca;
-        st[I_CAH * np + cell] = c.icah;
-        a.vd[nxt][cell] = vd;
-    } else if (wave == 1) {
-        const T Va = a.va[cur][cell];
-        c.k = st[CALCIUM_K * np + cell];
-        c.l = st[CALCIUM_L * np + cell];
-        c.h = st[SODIUM_H * np + cell];
-        c.n = st[POTASSIUM_N * np + cell];
-        c.p = st[POTASSIUM_P * np + cell];
-        c.xs = st[POTASSIUM_X_S * np + cell];
-        c.gcal = st[G_CAL * np + cell];
-        const T vs = soma<FM>(a.m, c, Vd, Vs, Va);
-        st[CALCIUM_K * np + cell] = c.k;
-        st[CALCIUM_L * np + cell] = c.l;
-        st[SODIUM_M * np + cell] = c.m;
-        st[SODIUM_H * np + cell] = c.h;
-        st[POTASSIUM_N * np + cell] = c.n;
-        st[POTASSIUM_P * np + cell] = c.p;
-        st[POTASSIUM_X_S * np + cell] = c.xs;
-        a.vs[nxt][cell] = vs;
-    } else {
-        const T Va = a.va[cur][cell];
-        c.ha = st[SODIUM_H_A * np + cell];
-        c.xa = st[POTASSIUM_X_A * np + cell];
-        const T va = axon<FM>(a.m, c, Vs, Va);
-        st[SODIUM_M_A * np + cell] = c.ma;
-        st[SODIUM_H_A * np + cell] = c.ha;
-        st[POTASSIUM_X_A * np + cell] = c.xa;
-        a.va[nxt][cell] = va;
-        if (a.trace && (t % a.stride) == 0)
-            a.trace[((t / a.stride) % a.trace_cap) * a.n + cell] = va;
+    if (wave == 0) tile_dend_cell<T, KAHAN, FM>(a, cell, t);
+    else if (wave == 1) tile_soma_cell<T, FM>(a, cell, t);
+    else tile_axon_cell<T, FM>(a, cell, t);
+}
+
+/* Dendrites only (their somata and axons ran beside the tiles), one thread
+ * per cell. */
+template <typename T, bool KAHAN>
+__global__ __launch_bounds__(kBlock) void k_tile_dend(StepArgs<T> a, int64_t k)
+{
+    const int64_t t = (a.d_step ? *a.d_step : 0) + k;
+    const int64_t cell = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (cell < a.n) tile_dend_cell<T, KAHAN, false>(a, cell, t);
+}
+
+/* One tile: copy the sender chunk's V_dend to LDS, then each receiver's
+ * junctions from that chunk (L lanes per receiver, list order) into a
+ * partial (sum, compensation) pair. */
+template <typename T, int L, bool KAHAN, int NT>
+__global__ __launch_bounds__(NT) void k_tile_gj(StepArgs<T> a, int64_t k)
+{
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
+    T *vch = reinterpret_cast<T *>(dyn_lds);
+    constexpr int CPW = 64 / L;
+    const int64_t t = (a.d_step ? *a.d_step : 0) + k;
+    if ((int)blockIdx.x >= a.ntiles) {
+        // workgroups past the tiles: somata (even waves) and axons (odd
+        // waves) of NT/2 cells, which need no gap-junction current, so they
+        // run beside the tiles
+        const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+        const int64_t cell = ((int64_t)(blockIdx.x - a.ntiles) * (NT / 128) + (wv >> 1)) * 64 + (threadIdx.x & 63);
+        if (cell < a.n) {
+            if (wv & 1) tile_axon_cell<T, false>(a, cell, t);
+            else tile_soma_cell<T, false>(a, cell, t);
+        }
+        return;
+    }
+    const T *__restrict__ vin = a.vd[t & 1];
+    const TileDesc td = a.tiles[blockIdx.x];
+    // the wavefront index is uniform: make it scalar so that the slice
+    // pointers and widths below live in SGPRs and the row loads use a
+    // scalar base with a 32-bit lane offset
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int cw = lane / L, l = lane % L;
+    const int64_t s0 = (int64_t)td.ch * a.ccells;
+    const int64_t cn = (a.n - s0 < a.ccells) ? a.n - s0 : a.ccells;
+    tile_chunk_to_lds<T, NT>(vin + s0, vch, cn, tid);
+    __syncthreads();
+    const int64_t cellb = (int64_t)td.rb * a.rcells;
+    constexpr int NW = NT / 64;
+    const int loff = cw * L + l;
+    // wavefront w takes slices w, 2NW-1-w, 2NW+w, ...: the tile's slices are
+    // ordered by width, so the snake evens out the waves' work.  The next
+    // slice's receiver entries are loaded while the current one is summed.
+    const auto snake = [&](int r) { return r * NW + ((r & 1) ? NW - 1 - wave : wave); };
+    int sl = snake(0), rloc = 0, d = 0;
+    if (sl < td.nsl) {
+        const int64_t slot = td.rl0 + (int64_t)sl * CPW + cw;
+        rloc = a.rlist[slot];
+        d = a.rdeg[slot];
+    }
+    for (int r = 0; r * NW < td.nsl; r++) {
+        const int sn = snake(r + 1);
+        int rloc_n = 0, d_n = 0;
+        if (sn < td.nsl) {
+            const int64_t slot = td.rl0 + (int64_t)sn * CPW + cw;
+            rloc_n = a.rlist[slot];
+            d_n = a.rdeg[slot];
+        }
+        if (sl < td.nsl) {
+            Acc<T, KAHAN> acc;
+            if (d > 0) {
+                const T vi = vin[cellb + rloc];
+                const int64_t base0 = a.tsp[td.sl0 + sl];
+                const int W = (int)((a.tsp[td.sl0 + sl + 1] - base0) >> 6);
+                if (td.wconst)
+                    tile_row_terms<T, KAHAN, L, true>(acc, vi, d, l, W, loff, a.tcol + base0, a.tw + base0,
+                                                      (T)td.wc, vch);
+                else
+                    tile_row_terms<T, KAHAN, L, false>(acc, vi, d, l, W, loff, a.tcol + base0, a.tw + base0,
+                                                       (T)td.wc, vch);
+            }
+            if constexpr (L > 1) {
+#pragma unroll
+                for (int off = L / 2; off > 0; off >>= 1) {
+                    T os = shfl_xor(acc.s, off);
+                    T oc = KAHAN ? shfl_xor(acc.c, off) : T(0);
+                    acc.merge(os, oc);
+                }
+            }
+            if (l == 0) part_store<T, KAHAN>(a.part, (int64_t)blockIdx.x * a.rcells + rloc, acc);
+        }
+        sl = sn;
+        rloc = rloc_n;
+        d = d_n;
     }
 }
 

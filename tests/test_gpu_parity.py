@@ -101,7 +101,7 @@ MODES = [(I.EXEC_STEP, "step"), (I.EXEC_GRAPH, "graph"), (I.EXEC_PERSISTENT, "pe
 GATHERS = [(I.GATHER_DIRECT, 76, "direct"), (I.GATHER_LDS, 76, "lds"),
            (I.GATHER_STAGED, 8, "staged8k"), (I.GATHER_STAGED, 76, "staged76k"),
            (I.GATHER_STAGED, 150, "staged150k"), (I.GATHER_TILED, 1, "tiled1k"),
-           (I.GATHER_TILED, 76, "tiled76k")]
+           (I.GATHER_TILED, 2, "tiled2k_sa"), (I.GATHER_TILED, 76, "tiled76k")]
 
 
 @pytest.mark.parametrize("mode,mname", MODES)
@@ -124,10 +124,11 @@ def test_exec_strategies_agree(mode, mname, L, gather, kb, gname, prec):
             net.close()
             return
         # 1 KiB chunks (256 FP32 / 128 FP64 senders) with 128-cell receiver
-        # blocks: several tiles per block, partials merged across chunks
+        # blocks: several tiles per block, partials merged across chunks;
+        # 2 KiB chunks with somata and axons run beside the tiles
         net.set_exec(mode=mode, lanes_per_cell=L, gather=gather, lds_kb=kb,
-                     tile_cells=128 if kb == 1 else 1024, steps_per_launch=50,
-                     block=256 if kb == 1 else 1024, ode_split=0 if kb == 1 else 1)
+                     tile_cells={1: 128, 2: 256}.get(kb, 1024), steps_per_launch=50,
+                     block={1: 256, 2: 512}.get(kb, 1024), ode_split={1: 0, 2: 2}.get(kb, 1))
     else:
         net.set_exec(mode=mode, lanes_per_cell=L, gather=gather, lds_kb=kb, steps_per_launch=50)
     out = net.advance(steps, trace_stride=10)
@@ -137,6 +138,39 @@ def test_exec_strategies_agree(mode, mname, L, gather, kb, gname, prec):
     tol = 1e-10 if prec == "fp64" else 2e-4
     assert np.abs(out - ref).max() < tol
     assert np.abs(vd - st[O.V_DEND]).max() < tol
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "fp32k"])
+@pytest.mark.parametrize("block", [256, 1024])
+def test_compartment_splits_bitwise_equal(prec, block):
+    """The three placements of the compartment update (one thread per cell;
+    one wavefront per compartment; somata and axons beside the tiles,
+    dendrites after) run the same arithmetic on the same start-of-step
+    values, so they agree bit for bit, trace included."""
+    say(f"ode_split {prec} block {block}")
+    n = 2000
+    rp, col, w, gcal, iapp = _random_case(n, p=0.05, seed=17)
+    outs = []
+    for split in (0, 1, 2):
+        net = make_net(n, prec, rp, col, w, gcal, iapp)
+        net.set_exec(mode=I.EXEC_GRAPH, gather=I.GATHER_TILED, lds_kb=2, tile_cells=512, block=block,
+                     lanes_per_cell=2, ode_split=split, steps_per_launch=40)
+        tr = net.advance(173, trace_stride=3)
+        outs.append((tr, np.stack([net.get_state(v) for v in I.VARS])))
+        net.close()
+    for tr, st in outs[1:]:
+        assert np.array_equal(tr, outs[0][0])
+        assert np.array_equal(st, outs[0][1])
+
+
+def test_compartment_split_options_checked():
+    say("ode_split option checks")
+    net = make_net(100, "fp32k")
+    with pytest.raises(I.InfoliError):
+        net.set_exec(gather=I.GATHER_TILED, ode_split=3)
+    with pytest.raises(I.InfoliError):
+        net.set_exec(gather=I.GATHER_TILED, ode_split=2, fast_math=1)
+    net.close()
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "fp32k"])
