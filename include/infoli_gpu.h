@@ -118,9 +118,28 @@ int infoli_set_connectivity_csr(infoli_t *ctx, const int64_t *row_ptr,
 
 /* Declare that the network is n_nets independent networks: cells
  * [offsets[k], offsets[k+1]) form network k and no neighbour list crosses a
- * boundary (checked).  Enables the batched kernel.  No reference
- * counterpart: the reference runs one network per process. */
+ * boundary (checked).  The networks then run as one block-diagonal
+ * network through the same kernels, many networks per launch.  No
+ * reference counterpart: the reference runs one network per process. */
 int infoli_set_partition(infoli_t *ctx, int n_nets, const int64_t *offsets);
+
+/* INFOLI_FP32_KAHAN only: cells with at least min_degree neighbours compute
+ * their gap-junction terms in FP64 (from the FP32 voltages and
+ * conductances, the term of src/single_threaded/infoli.c:548-553) and sum
+ * them in FP64; the others keep FP32 terms with compensated FP32 sums.
+ * 0 = no cell.  Default INFOLI_FP64_DEGREE_DEFAULT (0 for the other
+ * precisions, which reject a non-zero value).  No reference counterpart:
+ * the reference sums every cell in its one precision
+ * (src/openmp/infoli.c:496-520).
+ *
+ * The default is the degree at which compensated FP32 is expected to leave
+ * the 1e-3 mV budget (max |dV| against FP64 after 100 ms): measured
+ * 1.7e-4 mV at degree 100 (100,000 cells) and 8.2e-4 mV at degree 1000
+ * (10,000 cells), fitted as proportional to degree^0.68, reaching 1e-3 mV
+ * near degree 1260. */
+#define INFOLI_FP64_DEGREE_DEFAULT 1280
+int infoli_set_fp64_degree(infoli_t *ctx, int64_t min_degree);
+int64_t infoli_get_fp64_degree(const infoli_t *ctx);
 
 /* Per-cell g_CaL.  Replaces read_g_CaL_from_file (src/utils/utils.c:124-135)
  * under -DG_CAL_FROM_FILE (src/openmp/infoli.c:294-296). */
@@ -179,7 +198,6 @@ int infoli_gap_currents(infoli_t *ctx, double *i_c);
 #define INFOLI_EXEC_STEP        1  /* one fused kernel per step              */
 #define INFOLI_EXEC_GRAPH       2  /* per-step kernels replayed as hipGraph  */
 #define INFOLI_EXEC_PERSISTENT  3  /* multi-step kernel with grid barrier    */
-#define INFOLI_EXEC_BATCHED     4  /* per-network workgroups, LDS-resident V */
 
 /* How a step reads the neighbours' V_dend. */
 #define INFOLI_GATHER_DIRECT    0  /* random loads from the V_dend array        */

@@ -9,8 +9,8 @@ from .network import (  # noqa: F401
     FP32,
     FP32_KAHAN,
     FP64,
+    FP64_DEGREE_DEFAULT,
     EXEC_AUTO,
-    EXEC_BATCHED,
     EXEC_GRAPH,
     EXEC_PERSISTENT,
     EXEC_STEP,

@@ -111,6 +111,7 @@ struct infoli_ctx {
     std::vector<int32_t> col;
     std::vector<double> w;
     int max_deg = 0;
+    int64_t fp64_deg = 0;   // FP32 compensated: FP64 sums for cells of at least this degree
     std::vector<int64_t> part;
 
     infoli_exec exec{};
@@ -169,6 +170,12 @@ Model<double> make_model<double>(const infoli_params &p)
     return m;
 }
 
+/* some cell sums FP64 gap-junction terms (FP32 compensated, degree budget) */
+bool hyb_active(const infoli_ctx *c)
+{
+    return c->prec == INFOLI_FP32_KAHAN && c->fp64_deg > 0 && c->max_deg >= c->fp64_deg;
+}
+
 template <typename T>
 StepArgs<T> make_args(const infoli_ctx *c, int64_t stride)
 {
@@ -221,6 +228,7 @@ StepArgs<T> make_args(const infoli_ctx *c, int64_t stride)
         a.tw = (const T *)c->d_tw;
         a.part = c->d_part;
         a.ntiles = (int32_t)c->tl_ntiles;
+        a.fp64_deg = hyb_active(c) ? (int32_t)std::min<int64_t>(c->fp64_deg, INT32_MAX) : 0;
         a.rcells = c->tiled_R;
         a.ccells = (int32_t)c->tl_C;
     }
@@ -289,6 +297,13 @@ const void *const k_tile_ode3_tab[2][3] = {
     {(const void *)k_tile_ode3<double, false, false>, (const void *)k_tile_ode3<float, false, true>,
      (const void *)k_tile_ode3<float, true, true>},
 };
+/* FP32 compensated with FP64 sums for high-degree cells: [threads][L] */
+const void *const k_tile_gj_hyb_tab[3][5] = {
+    INFOLI_L_TABLE(k_tile_gj, float, true, 256, true),
+    INFOLI_L_TABLE(k_tile_gj, float, true, 512, true),
+    INFOLI_L_TABLE(k_tile_gj, float, true, 1024, true),
+};
+const void *const k_gap_hyb_tab[5] = INFOLI_L_TABLE(k_gap_currents, float, true, true);
 const void *const k_tile_dend_tab[3] = {
     (const void *)k_tile_dend<double, false>, (const void *)k_tile_dend<float, false>,
     (const void *)k_tile_dend<float, true>,
@@ -326,7 +341,8 @@ int64_t n_blocks(const infoli_ctx *c)
 int launch_step(infoli_ctx *c, const void *args, int64_t k)
 {
     if (c->exec.gather == GATHER_TILED) {
-        const void *fg = k_tile_gj_tab[c->prec][ntidx(c->exec.block)][lidx(c->exec.lanes_per_cell)];
+        const void *fg = hyb_active(c) ? k_tile_gj_hyb_tab[ntidx(c->exec.block)][lidx(c->exec.lanes_per_cell)]
+                                       : k_tile_gj_tab[c->prec][ntidx(c->exec.block)][lidx(c->exec.lanes_per_cell)];
         const int fm = c->exec.fast_math ? 1 : 0;
         const void *fo = k_tile_ode_tab[fm][c->prec];
         int64_t kk = k;
@@ -763,7 +779,7 @@ void default_exec(infoli_ctx *c)
     c->exec.ode_split = 1;
     c->exec.fast_math = 0;
     c->exec.gather = GATHER_DIRECT;
-    if (mean_deg < 4) return;
+    if (mean_deg < 4 && !hyb_active(c)) return;
     // tiled: lanes per receiver from the mean in-tile degree of a first
     // layout guess (about 16 junctions per lane), receivers per tile
     // inversely, sender chunks small enough for one tile per CU (two when
@@ -791,7 +807,7 @@ void default_exec(infoli_ctx *c)
     const int L = lanes_for(in_tile_degree(R, Ck));
     const double psz = c->prec == INFOLI_FP32 ? 4.0 : 8.0;
     const double tpb = tiles_per_block(c, R, Ck);
-    if (tpb * psz <= 0.5 * mean_deg * (2.0 + (double)c->tsize)) {
+    if (hyb_active(c) || tpb * psz <= 0.5 * mean_deg * (2.0 + (double)c->tsize)) {
         c->exec.gather = GATHER_TILED;
         c->exec.lds_kb = kb;
         c->exec.block = 1024;
@@ -900,6 +916,7 @@ extern "C" int infoli_create(infoli_t **out, const infoli_params *params, int64_
     c->n = n_cells;
     c->npad = round_up(n_cells, kBlock);
     c->tsize = precision == INFOLI_FP64 ? 8 : 4;
+    c->fp64_deg = precision == INFOLI_FP32_KAHAN ? INFOLI_FP64_DEGREE_DEFAULT : 0;
     c->params = *params;
     c->num_cus = prop.multiProcessorCount;
     c->row_ptr.assign(n_cells + 1, 0);
@@ -1019,6 +1036,32 @@ extern "C" int infoli_set_connectivity_csr(infoli_t *c, const int64_t *row_ptr,
     return ensure_layout(c);
 }
 
+extern "C" int infoli_set_fp64_degree(infoli_t *c, int64_t min_degree)
+{
+    infoli_clear_error();
+    RCHK(validate_ctx(c, "infoli_set_fp64_degree"));
+    if (min_degree < 0) return infoli_fail(INFOLI_ERR_ARG, "infoli_set_fp64_degree: negative degree");
+    if (min_degree != 0 && c->prec != INFOLI_FP32_KAHAN)
+        return infoli_fail(INFOLI_ERR_ARG, "infoli_set_fp64_degree: FP64 sums for high-degree cells apply "
+                                           "to INFOLI_FP32_KAHAN only");
+    if (min_degree == c->fp64_deg) return INFOLI_OK;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->fp64_deg = min_degree;
+    if (hyb_active(c) && c->exec.gather != GATHER_TILED) {
+        const int keep_mode = c->exec.mode;
+        default_exec(c);
+        if (keep_mode != INFOLI_EXEC_PERSISTENT) c->exec.mode = keep_mode;
+    }
+    drop_graph(c);
+    return ensure_layout(c);
+}
+
+extern "C" int64_t infoli_get_fp64_degree(const infoli_t *c)
+{
+    return c ? c->fp64_deg : -1;
+}
+
 extern "C" int infoli_set_partition(infoli_t *c, int n_nets, const int64_t *offsets)
 {
     infoli_clear_error();
@@ -1136,24 +1179,11 @@ extern "C" int infoli_get_state(infoli_t *c, int var, double *values)
 /* execution                                                                 */
 /* ------------------------------------------------------------------------ */
 
-extern "C" int infoli_get_exec(const infoli_t *c, infoli_exec *out)
+/* Validate an execution strategy for this network (normalising the block
+ * size of the non-tiled gathers); INFOLI_OK or INFOLI_ERR_ARG with the
+ * reason in the error message. */
+int check_exec(const infoli_ctx *c, infoli_exec &x)
 {
-    RCHK(validate_ctx(c, "infoli_get_exec"));
-    if (!out) return infoli_fail(INFOLI_ERR_ARG, "infoli_get_exec: null output");
-    *out = c->exec;
-    return INFOLI_OK;
-}
-
-extern "C" int infoli_set_exec(infoli_t *c, const infoli_exec *e)
-{
-    infoli_clear_error();
-    RCHK(validate_ctx(c, "infoli_set_exec"));
-    if (!e) return infoli_fail(INFOLI_ERR_ARG, "infoli_set_exec: null exec");
-    infoli_exec x = *e;
-    if (x.mode == INFOLI_EXEC_AUTO) {
-        default_exec(c);
-        x = c->exec;
-    }
     if (x.mode < INFOLI_EXEC_STEP || x.mode > INFOLI_EXEC_PERSISTENT)
         return infoli_fail(INFOLI_ERR_ARG, "infoli_set_exec: mode %d not available", x.mode);
     if (lidx(x.lanes_per_cell) < 0)
@@ -1174,6 +1204,10 @@ extern "C" int infoli_set_exec(infoli_t *c, const infoli_exec *e)
         return infoli_fail(INFOLI_ERR_ARG, "ode_split %d not in {0, 1, 2}", x.ode_split);
     if (x.fast_math && x.ode_split == 2)
         return infoli_fail(INFOLI_ERR_ARG, "fast_math is available with ode_split 0 or 1 only");
+    if (hyb_active(c) && x.gather != GATHER_TILED)
+        return infoli_fail(INFOLI_ERR_ARG, "cells with at least %lld neighbours sum FP64 gap-junction terms, "
+                                           "which the tiled gather implements; set the FP64 degree to 0 "
+                                           "for another gather", (long long)c->fp64_deg);
     if (x.gather == GATHER_TILED && x.mode == INFOLI_EXEC_PERSISTENT)
         return infoli_fail(INFOLI_ERR_ARG, "the tiled gather runs as two kernels per step; "
                                            "use the step or graph launch mode");
@@ -1188,6 +1222,28 @@ extern "C" int infoli_set_exec(infoli_t *c, const infoli_exec *e)
         ((int64_t)x.lds_kb * 1024 + (int64_t)(c->max_deg + 64 + kBlock) * (int64_t)c->tsize) > 160 * 1024)
         return infoli_fail(INFOLI_ERR_ARG, "staged gather: %d KiB plus the row slack of degree %d exceeds "
                                            "the 160 KiB of LDS", x.lds_kb, c->max_deg);
+    return INFOLI_OK;
+}
+
+extern "C" int infoli_get_exec(const infoli_t *c, infoli_exec *out)
+{
+    RCHK(validate_ctx(c, "infoli_get_exec"));
+    if (!out) return infoli_fail(INFOLI_ERR_ARG, "infoli_get_exec: null output");
+    *out = c->exec;
+    return INFOLI_OK;
+}
+
+extern "C" int infoli_set_exec(infoli_t *c, const infoli_exec *e)
+{
+    infoli_clear_error();
+    RCHK(validate_ctx(c, "infoli_set_exec"));
+    if (!e) return infoli_fail(INFOLI_ERR_ARG, "infoli_set_exec: null exec");
+    infoli_exec x = *e;
+    if (x.mode == INFOLI_EXEC_AUTO) {
+        default_exec(c);
+        x = c->exec;
+    }
+    RCHK(check_exec(c, x));
     const infoli_exec prev = c->exec;
     c->exec = x;
     drop_graph(c);
@@ -1437,7 +1493,8 @@ extern "C" int infoli_gap_currents(infoli_t *c, double *i_c)
     const int64_t nblk = c->npad / kBlock;
     void *out = nullptr;
     HIPCHK(hipMalloc(&out, (size_t)c->n * c->tsize));
-    const void *fn = k_gap_tab[c->prec][lidx(c->exec.lanes_per_cell)];
+    const void *fn = hyb_active(c) ? k_gap_hyb_tab[lidx(c->exec.lanes_per_cell)]
+                                   : k_gap_tab[c->prec][lidx(c->exec.lanes_per_cell)];
     int64_t t = c->step;
     int rc = INFOLI_OK;
     if (is_f32(c)) {
@@ -1495,28 +1552,51 @@ extern "C" int infoli_autotune(infoli_t *c, int64_t trial_steps, infoli_exec *ch
     const bool prof = c->profiling;
     c->profiling = false;
 
+    // candidates: the heuristic's choice first, then tiled shapes around it
+    // (lanes, receivers per tile and chunk size halved and doubled, both
+    // compartment placements), then the other gathers unless some cell
+    // needs the FP64 sums only the tiled gather has
     std::vector<infoli_exec> cands;
-    const int Ls[] = {1, 2, 4, 8, 16};
-    const int modes[] = {INFOLI_EXEC_GRAPH, INFOLI_EXEC_PERSISTENT};
-    const int kbs[] = {40, 76, 150};
     const int spl = (int)std::min<int64_t>(trial_steps, 500);
-    for (int L : Ls)
-        for (int mode : modes) {
-            cands.push_back(infoli_exec{mode, L, kBlock, spl, GATHER_DIRECT, kDefaultLdsKb, 1024, 1, 0});
-            if ((size_t)c->n * c->tsize <= kVldsMaxBytes)
-                cands.push_back(infoli_exec{mode, L, kBlock, spl, GATHER_VLDS, kDefaultLdsKb, 1024, 1, 0});
-            for (int kb : kbs)
-                if (max_block_entries_fit(c, kb))
-                    cands.push_back(infoli_exec{mode, L, kBlock, spl, GATHER_STAGED, kb, 1024, 1, 0});
-            if (mode == INFOLI_EXEC_GRAPH && L <= 2)
-                for (int kb : {32, 48})
-                    for (int R : {1024, 2048})
-                        cands.push_back(infoli_exec{mode, L, 1024, spl, GATHER_TILED, kb, R, 1, 0});
+    default_exec(c);
+    infoli_exec h = c->exec;
+    c->exec = orig;
+    h.steps_per_launch = spl;
+    cands.push_back(h);
+    const int L0 = h.gather == GATHER_TILED ? h.lanes_per_cell : 1;
+    const int R0 = h.gather == GATHER_TILED ? h.tile_cells : 2048;
+    const int kb0 = h.gather == GATHER_TILED ? h.lds_kb : 48;
+    for (int L : {L0 / 2, L0, L0 * 2}) {
+        if (lidx(L) < 0) continue;
+        for (int R : {R0 / 2, R0, R0 * 2}) {
+            if (R < 64 || R > 16384 || R % 64) continue;
+            for (int kb : {std::max(1, kb0 / 2), kb0}) {
+                if (kb > 156) continue;
+                for (int split : {1, 2}) {
+                    const infoli_exec e{INFOLI_EXEC_GRAPH, L, 1024, spl, GATHER_TILED, kb, R, split, 0};
+                    if (memcmp(&e, &h, sizeof e) != 0) cands.push_back(e);
+                }
+            }
         }
+    }
+    if (!hyb_active(c)) {
+        const int modes[] = {INFOLI_EXEC_GRAPH, INFOLI_EXEC_PERSISTENT};
+        for (int L : {1, 2, 4, 8, 16})
+            for (int mode : modes) {
+                if (mode == INFOLI_EXEC_PERSISTENT && L != 4) continue;
+                cands.push_back(infoli_exec{mode, L, kBlock, spl, GATHER_DIRECT, kDefaultLdsKb, 1024, 1, 0});
+                if ((size_t)c->n * c->tsize <= kVldsMaxBytes)
+                    cands.push_back(infoli_exec{mode, L, kBlock, spl, GATHER_VLDS, kDefaultLdsKb, 1024, 1, 0});
+                for (int kb : {40, 76, 150})
+                    if (max_block_entries_fit(c, kb))
+                        cands.push_back(infoli_exec{mode, L, kBlock, spl, GATHER_STAGED, kb, 1024, 1, 0});
+            }
+    }
     double best = 1e300;
     infoli_exec best_e = orig;
     int rc = INFOLI_OK;
-    for (const infoli_exec &e : cands) {
+    for (infoli_exec e : cands) {
+        if (check_exec(c, e) != INFOLI_OK) continue;
         c->exec = e;
         drop_graph(c);
         rc = ensure_layout(c);
@@ -1542,6 +1622,7 @@ extern "C" int infoli_autotune(infoli_t *c, int64_t trial_steps, infoli_exec *ch
     c->profiling = prof;
     RCHK(ensure_layout(c));
     if (rc != INFOLI_OK) return infoli_fail(rc, "infoli_autotune: a trial failed (%s)", infoli_strerror(rc));
+    infoli_clear_error();   // the messages of candidates that did not apply
     if (chosen) *chosen = best_e;
     return INFOLI_OK;
 }

@@ -87,6 +87,8 @@ struct StepArgs {
     const uint16_t *tcol;      // sender index local to the chunk
     const T *tw;               // conductance
     void *part;                // partial sums, [tile][receiver block size] (sum, comp) pairs
+    int32_t fp64_deg;          // FP32 compensated: cells with at least this many
+                               // neighbours sum FP64 terms (0 = none)
     int32_t ntiles;            // tiles (workgroups past them run somata and axons)
     int32_t rcells;            // receivers per block (multiple of 64)
     int32_t ccells;            // senders per chunk
@@ -810,7 +812,7 @@ __global__ __launch_bounds__(kBlock) void k_persist(StepArgs<T> a, GridBarrier *
 
 /* I_c of every cell from V_dend at step t (no state change); direct gathers
  * over the uniform 256-cell blocks of the sliced-ELL layout. */
-template <typename T, int L, bool KAHAN>
+template <typename T, int L, bool KAHAN, bool HYB = false>
 __global__ __launch_bounds__(kBlock) void k_gap_currents(StepArgs<T> a, int64_t t, T *out)
 {
     constexpr int CPW = 64 / L;
@@ -829,9 +831,19 @@ __global__ __launch_bounds__(kBlock) void k_gap_currents(StepArgs<T> a, int64_t 
             const int d = a.deg[cell];
             const T vi = vin[cell];
             const int64_t base = a.slice_ptr[slice] + cw * L + l;
-            for (int kk = 0; kk * L + l < d; kk++) {
-                const int64_t e = base + (int64_t)kk * 64;
-                acc.add(gj_term(vi, vin[a.col[e]], a.w[e]));
+            if (HYB && d >= a.fp64_deg) {
+                double sd = 0.0;
+                for (int kk = 0; kk * L + l < d; kk++) {
+                    const int64_t e = base + (int64_t)kk * 64;
+                    sd += gj_term((double)vi, (double)vin[a.col[e]], (double)a.w[e]);
+                }
+                acc.s = (T)sd;
+                acc.c = (T)(sd - (double)acc.s);
+            } else {
+                for (int kk = 0; kk * L + l < d; kk++) {
+                    const int64_t e = base + (int64_t)kk * 64;
+                    acc.add(gj_term(vi, vin[a.col[e]], a.w[e]));
+                }
             }
         }
         if constexpr (L > 1) {
@@ -899,10 +911,10 @@ __device__ __forceinline__ X ld_off(const X *__restrict__ p, uint32_t byte_off)
  * irow and wrow uniform across the wavefront, W the slice width.  Same
  * arithmetic and order as row_terms_idx; WC: every junction has
  * conductance wc. */
-template <typename T, bool KAHAN, int L, bool WC>
-__device__ __forceinline__ void tile_row_terms(Acc<T, KAHAN> &acc, T vi, int d, int l, int W, int loff,
-                                               const uint16_t *__restrict__ irow, const T *__restrict__ wrow,
-                                               T wc, const T *__restrict__ vch)
+template <typename T, bool KAHAN, int L, bool WC, bool HYB>
+__device__ __forceinline__ void tile_row_terms(Acc<T, KAHAN> &acc, double &accd, bool wide, T vi, int d, int l,
+                                               int W, int loff, const uint16_t *__restrict__ irow,
+                                               const T *__restrict__ wrow, T wc, const T *__restrict__ vch)
 {
     const uint32_t oi = (uint32_t)loff * (uint32_t)sizeof(uint16_t);
     const uint32_t ow = (uint32_t)loff * (uint32_t)sizeof(T);
@@ -921,11 +933,21 @@ __device__ __forceinline__ void tile_row_terms(Acc<T, KAHAN> &acc, T vi, int d, 
 #pragma unroll
         for (int u = 0; u < kUnroll; u++)
             if (kk0 + u < W) vv[u] = vch[jv[u]];
+        if (HYB && wide) {
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-            if (kk0 + u < W) {
-                const T term = gj_term(vi, vv[u], WC ? wc : wv[u]);
-                acc.add(((kk0 + u) * L + l < d) ? term : T(0));
+            for (int u = 0; u < kUnroll; u++) {
+                if (kk0 + u < W) {
+                    const double term = gj_term((double)vi, (double)vv[u], (double)(WC ? wc : wv[u]));
+                    accd += ((kk0 + u) * L + l < d) ? term : 0.0;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) {
+                if (kk0 + u < W) {
+                    const T term = gj_term(vi, vv[u], WC ? wc : wv[u]);
+                    acc.add(((kk0 + u) * L + l < d) ? term : T(0));
+                }
             }
         }
     }
@@ -1085,7 +1107,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_dend(StepArgs<T> a, int64_t k)
 /* One tile: copy the sender chunk's V_dend to LDS, then each receiver's
  * junctions from that chunk (L lanes per receiver, list order) into a
  * partial (sum, compensation) pair. */
-template <typename T, int L, bool KAHAN, int NT>
+template <typename T, int L, bool KAHAN, int NT, bool HYB = false>
 __global__ __launch_bounds__(NT) void k_tile_gj(StepArgs<T> a, int64_t k)
 {
     extern __shared__ __align__(16) unsigned char dyn_lds[];
@@ -1142,12 +1164,20 @@ __global__ __launch_bounds__(NT) void k_tile_gj(StepArgs<T> a, int64_t k)
                 const T vi = vin[cellb + rloc];
                 const int64_t base0 = a.tsp[td.sl0 + sl];
                 const int W = (int)((a.tsp[td.sl0 + sl + 1] - base0) >> 6);
+                // high-degree receivers: FP64 terms and sum, returned as a
+                // (sum, compensation) pair of floats
+                const bool wide = HYB && a.deg[cellb + rloc] >= a.fp64_deg;
+                double accd = 0.0;
                 if (td.wconst)
-                    tile_row_terms<T, KAHAN, L, true>(acc, vi, d, l, W, loff, a.tcol + base0, a.tw + base0,
-                                                      (T)td.wc, vch);
+                    tile_row_terms<T, KAHAN, L, true, HYB>(acc, accd, wide, vi, d, l, W, loff, a.tcol + base0,
+                                                           a.tw + base0, (T)td.wc, vch);
                 else
-                    tile_row_terms<T, KAHAN, L, false>(acc, vi, d, l, W, loff, a.tcol + base0, a.tw + base0,
-                                                       (T)td.wc, vch);
+                    tile_row_terms<T, KAHAN, L, false, HYB>(acc, accd, wide, vi, d, l, W, loff, a.tcol + base0,
+                                                            a.tw + base0, (T)td.wc, vch);
+                if (HYB && wide) {
+                    acc.s = (T)accd;
+                    acc.c = (T)(accd - (double)acc.s);
+                }
             }
             if constexpr (L > 1) {
 #pragma unroll

@@ -50,6 +50,7 @@ static void print_usage(void)
     printf("-o denotes a filename for simulation output. Default value: InferiorOlive_Output.txt.\n");
     printf("GPU build options: -g <gcal file> -m <conn matrix> -r <conn triplets> -x fp32|fp32k|fp64\n");
     printf("                   -s <seed> -k <trace stride> -D <device> -a (autotune)\n");
+    printf("                   -b <degree> (fp32k: FP64 sums for cells of at least this degree; 0 = none)\n");
 }
 
 static void die(int rc, const char *what)
@@ -76,6 +77,7 @@ int main(int argc, char **argv)
     uint64_t seed = (uint64_t)time(NULL);
     int64_t stride = 100;
     int stride_given = 0;
+    int64_t fp64_degree = -1;   /* -1: the library's default */
 
     /* config next to the executable (OMP:113-121) */
     ssize_t len = readlink("/proc/self/exe", params_file, PATH_MAX);
@@ -89,7 +91,7 @@ int main(int argc, char **argv)
     strcat(params_file, "default.conf");
 
     int opt;
-    while ((opt = getopt(argc, argv, "hn:p:t:i:c:o:g:m:r:x:s:k:D:a")) != -1) {
+    while ((opt = getopt(argc, argv, "hn:p:t:i:c:o:g:m:r:x:s:k:D:ab:")) != -1) {
         switch (opt) {
         case 'h': print_usage(); return 0;
         case 'n':
@@ -141,6 +143,10 @@ int main(int argc, char **argv)
             break;
         case 'D': device = atoi(optarg); break;
         case 'a': autotune = 1; break;
+        case 'b':
+            fp64_degree = strtoll(optarg, NULL, 10);
+            if (fp64_degree < 0) { print_usage(); exit(EXIT_FAILURE); }
+            break;
         default: print_usage(); exit(EXIT_FAILURE);
         }
     }
@@ -182,6 +188,10 @@ int main(int argc, char **argv)
     infoli_t *net = NULL;
     rc = infoli_create(&net, &params, n, precision, device);
     if (rc) die(rc, "create");
+    if (fp64_degree >= 0) {
+        rc = infoli_set_fp64_degree(net, fp64_degree);
+        if (rc) die(rc, "FP64 degree");
+    }
     rc = infoli_set_connectivity_csr(net, row_ptr, col, w);
     if (rc) die(rc, "connectivity");
     free(row_ptr); free(col); free(w);

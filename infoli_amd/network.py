@@ -14,9 +14,10 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "lib", "libinfoli_gpu.so")
 
 FP64, FP32, FP32_KAHAN = 0, 1, 2
+FP64_DEGREE_DEFAULT = 1280   # FP32_KAHAN: cells of at least this degree sum FP64 terms
 _PREC_NAMES = {"fp64": FP64, "fp32": FP32, "fp32k": FP32_KAHAN, "fp32_kahan": FP32_KAHAN}
 
-EXEC_AUTO, EXEC_STEP, EXEC_GRAPH, EXEC_PERSISTENT, EXEC_BATCHED = 0, 1, 2, 3, 4
+EXEC_AUTO, EXEC_STEP, EXEC_GRAPH, EXEC_PERSISTENT = 0, 1, 2, 3
 GATHER_DIRECT, GATHER_LDS, GATHER_STAGED, GATHER_TILED = 0, 1, 2, 3
 
 # per-cell variables, bin/default.conf lines 24-42 order
@@ -72,6 +73,8 @@ def library():
         "infoli_destroy": ([P], None),
         "infoli_set_connectivity_csr": ([P, P, P, P], i32),
         "infoli_set_partition": ([P, i32, P], i32),
+        "infoli_set_fp64_degree": ([P, i64], i32),
+        "infoli_get_fp64_degree": ([P], i64),
         "infoli_set_gcal": ([P, P], i32),
         "infoli_set_iapp": ([P, P], i32),
         "infoli_set_iapp_pulse": ([P, P, d, i64, i64], i32),
@@ -214,6 +217,15 @@ class Network:
         offsets = np.ascontiguousarray(offsets, np.int64)
         _check(library().infoli_set_partition(self._h, len(offsets) - 1, _p(offsets)),
                "set_partition")
+
+    def set_fp64_degree(self, min_degree):
+        """FP32 compensated only: cells with at least `min_degree` neighbours
+        compute and sum their gap-junction terms in FP64 (0 = none)."""
+        _check(library().infoli_set_fp64_degree(self._h, int(min_degree)), "set_fp64_degree")
+
+    @property
+    def fp64_degree(self):
+        return int(library().infoli_get_fp64_degree(self._h))
 
     def _cells(self, values, dtype=np.float64):
         a = np.ascontiguousarray(values, dtype)

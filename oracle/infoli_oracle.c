@@ -17,7 +17,9 @@
  *                   literals, in place, voltages committed last (:673-675).
  *   ORACLE_FP32K -- ORACLE_FP32 with the per-cell neighbour sum compensated:
  *                   each term enters a TwoSum (Knuth) accumulator in list
- *                   order; the result is sum + compensation.
+ *                   order; the result is sum + compensation.  Cells at or
+ *                   above oracle_set_fp64_degree()'s budget (default: none)
+ *                   take ST's FP64 term on the FP32 values, summed in FP64.
  *
  * Build: gcc -O2 -fopenmp -ffp-contract=off (no FMA contraction, no
  * fast-math) so the operation order below is the order executed.
@@ -30,7 +32,9 @@
  *     measured) and tracks src/regression_test/stable_version.c's baseline
  *     (float state, double exp) within the reference checker's 1% criterion;
  *   - gap-junction sums match a long-double brute force, are odd in
- *     V_i - V_j and conserve sum_i I_c = 0 for symmetric w.
+ *     V_i - V_j and conserve sum_i I_c = 0 for symmetric w;
+ *   - under a degree budget, FP32K cells at or above it equal the FP64
+ *     sum (float)gj64 bit for bit on float-exact inputs.
  * Every exported function is pinned by at least one of these.
  */
 #include <math.h>
@@ -259,6 +263,35 @@ static float gj32k(const float *vd, int64_t i, const int64_t *rp,
     return sum + comp;
 }
 
+/* Cells with at least this many neighbours take FP64 terms and an FP64 sum
+ * in ORACLE_FP32K (0 = none): the product's degree budget, restated as
+ * ST's FP64 term (ST:548-553) applied to the FP32 voltages and the FP32
+ * conductances, summed in list order and rounded to float. */
+static int64_t g_fp64_degree = 0;
+
+void oracle_set_fp64_degree(int64_t d) { g_fp64_degree = d < 0 ? 0 : d; }
+
+static float gj32w(const float *vd, int64_t i, const int64_t *rp,
+                   const int32_t *col, const float *w)
+{
+    double sum = 0;
+    for (int64_t e = rp[i]; e < rp[i + 1]; e++) {
+        double V = (double)vd[i] - (double)vd[col[e]];
+        sum += (double)w[e] * (0.8 * exp(-1 * (V * V) / 100) + 0.2) * V;
+    }
+    return (float)sum;
+}
+
+/* ORACLE_FP32K neighbour current: compensated FP32, or FP64 at or above the
+ * degree budget */
+static float gj32k_cell(const float *vd, int64_t i, const int64_t *rp,
+                        const int32_t *col, const float *w)
+{
+    if (g_fp64_degree > 0 && rp[i + 1] - rp[i] >= g_fp64_degree)
+        return gj32w(vd, i, rp, col, w);
+    return gj32k(vd, i, rp, col, w);
+}
+
 /* compartment loop body, OMP:556-676, one cell, in place */
 static void cell32(const p32 *P, float **s, int64_t i, float I_c, float iApp)
 {
@@ -470,7 +503,7 @@ int oracle_run(int prec, const double *conf, const float *conff, int64_t n,
             if (prec == ORACLE_FP32K) {
                 #pragma omp parallel for schedule(static) num_threads(nthreads)
                 for (int64_t i = 0; i < n; i++)
-                    ic[i] = (nnz > 0) ? gj32k(s[V_DEND], i, rp, col, wf) : 0.0f;
+                    ic[i] = (nnz > 0) ? gj32k_cell(s[V_DEND], i, rp, col, wf) : 0.0f;
             } else {
                 #pragma omp parallel for schedule(static) num_threads(nthreads)
                 for (int64_t i = 0; i < n; i++)
@@ -517,7 +550,7 @@ int oracle_gap_currents(int prec, int64_t n, const int64_t *rp,
     for (int64_t i = 0; i < n; i++) vf[i] = (float)vdend[i];
     for (int64_t e = 0; e < nnz; e++) wf[e] = (float)w[e];
     for (int64_t i = 0; i < n; i++)
-        ic[i] = (prec == ORACLE_FP32K) ? gj32k(vf, i, rp, col, wf)
+        ic[i] = (prec == ORACLE_FP32K) ? gj32k_cell(vf, i, rp, col, wf)
                                        : gj32(vf, i, rp, col, wf);
     free(vf); free(wf);
     return 0;

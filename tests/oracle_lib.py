@@ -64,6 +64,8 @@ def lib():
         L.oracle_gap_currents.argtypes = [ctypes.c_int, i64, P, P, P, P, P]
         L.oracle_gap_currents.restype = ctypes.c_int
         L.oracle_max_threads.restype = ctypes.c_int
+        L.oracle_set_fp64_degree.argtypes = [i64]
+        L.oracle_set_fp64_degree.restype = None
         _lib = L
     return _lib
 
@@ -149,3 +151,8 @@ def dense_to_csr(mat, threshold=0.0):
                 ws.append(v)
         rows.append(len(cols))
     return (np.array(rows, np.int64), np.array(cols, np.int32), np.array(ws, np.float64))
+
+
+def set_fp64_degree(d):
+    """FP32K cells with at least d neighbours sum FP64 terms (0 = none)."""
+    lib().oracle_set_fp64_degree(int(d))

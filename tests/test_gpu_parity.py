@@ -45,7 +45,11 @@ def make_net(n, prec, rp=None, col=None, w=None, gcal=None, iapp=None, **exec_kw
     return net
 
 
-def oracle_ref(prec, n, steps, rp=None, col=None, w=None, gcal=None, iapp=None, **kw):
+def oracle_ref(prec, n, steps, rp=None, col=None, w=None, gcal=None, iapp=None, fp64_degree=None, **kw):
+    # the library's default degree budget for FP32 compensated sums
+    if fp64_degree is None:
+        fp64_degree = I.FP64_DEGREE_DEFAULT if prec == "fp32k" else 0
+    O.set_fp64_degree(fp64_degree)
     st = O.initial_state(n)
     if gcal is not None:
         st[O.G_CAL] = gcal
@@ -171,6 +175,100 @@ def test_compartment_split_options_checked():
     with pytest.raises(I.InfoliError):
         net.set_exec(gather=I.GATHER_TILED, ode_split=2, fast_math=1)
     net.close()
+
+
+def _mixed_degree_case(n=1200, seed=23):
+    """Directed graph whose in-degrees spread over [20, 140): a degree budget
+    of 80 splits the cells, and the wavefronts, into both kinds."""
+    rng = np.random.default_rng(seed)
+    rows = [np.sort(rng.choice(np.delete(np.arange(n), i), size=rng.integers(20, 140), replace=False))
+            for i in range(n)]
+    rp = np.zeros(n + 1, np.int64)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    col = np.concatenate(rows).astype(np.int32)
+    w = rng.uniform(0.02, 0.06, len(col))
+    return rp, col, w
+
+
+@pytest.mark.parametrize("L", [1, 4, 16])
+def test_fp64_degree_budget_gap_currents(L):
+    """FP32 compensated with a degree budget: cells at or above it sum FP64
+    terms; I_c matches the oracle's restatement within one float ulp plus the
+    FP64 reassociation of the sum, the other cells are untouched."""
+    say(f"fp64 degree budget gap currents L={L}")
+    n = 1200
+    rp, col, w = _mixed_degree_case(n)
+    deg = np.diff(rp)
+    v = np.random.default_rng(5).uniform(-70, 10, n).astype(np.float32).astype(np.float64)
+    net = make_net(n, "fp32k", rp, col, w, lanes_per_cell=L)
+    net.set_fp64_degree(80)
+    net.set_state("V_dend", v)
+    ic = net.gap_currents()
+    net.set_fp64_degree(0)
+    ic0 = net.gap_currents()
+    net.close()
+    O.set_fp64_degree(80)
+    ref = O.gap_currents(O.FP32K, v, rp, col, w)
+    O.set_fp64_degree(0)
+    ref0 = O.gap_currents(O.FP32K, v, rp, col, w)
+    wide = deg >= 80
+    assert 0.2 < wide.mean() < 0.8
+    rowid = np.repeat(np.arange(n), deg)
+    dv = v[rowid] - v[col]
+    absum = np.bincount(rowid, np.abs(w * (0.8 * np.exp(-dv * dv / 100) + 0.2) * dv), n)
+    ulp = np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64)
+    assert (np.abs(ic - ref)[wide] <= ulp[wide] + 1e-12 * absum[wide] + 1e-30).all()
+    # the FP64 sums are closer to the FP64 oracle than the compensated FP32 ones
+    ref64 = O.gap_currents(O.FP64, v, rp, col, w.astype(np.float32).astype(np.float64))
+    assert np.abs(ic - ref64)[wide].max() <= np.abs(ic0 - ref64)[wide].max()
+    # below the budget: the compensated FP32 path, as with no budget
+    assert np.array_equal(ic[~wide], ic0[~wide])
+
+
+@pytest.mark.parametrize("ode_split", [1, 2])
+def test_fp64_degree_budget_run_vs_oracle(ode_split):
+    say(f"fp64 degree budget run split={ode_split}")
+    n, steps = 1200, 300
+    rp, col, w = _mixed_degree_case(n)
+    gcal = I.gen_uniform(7, 2, n, 0.5, 1.1)
+    iapp = I.gen_uniform(7, 3, n, -1.0, 3.0)
+    net = make_net(n, "fp32k", rp, col, w, gcal, iapp)
+    net.set_fp64_degree(80)
+    net.set_exec(mode=I.EXEC_GRAPH, gather=I.GATHER_TILED, lds_kb=1, tile_cells=256, block=512,
+                 lanes_per_cell=2, ode_split=ode_split, steps_per_launch=50)
+    out = net.advance(steps, trace_stride=10)
+    net.close()
+    ref, _ = oracle_ref("fp32k", n, steps, rp, col, w, gcal, iapp, fp64_degree=80, stride=10)
+    assert np.abs(out - ref).max() < 2e-4
+
+
+def test_fp64_degree_budget_options():
+    say("fp64 degree options")
+    n = 1301
+    rows = [np.arange(1, n)] + [np.array([0])] * (n - 1)
+    rp = np.zeros(n + 1, np.int64)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    col = np.concatenate(rows).astype(np.int32)
+    w = np.full(len(col), 0.05)
+    net = make_net(n, "fp32k", rp, col, w)
+    assert net.fp64_degree == I.FP64_DEGREE_DEFAULT
+    # the hub's 1300 junctions exceed the budget: the tiled gather is chosen
+    # and the others are refused until the budget is lifted
+    assert net.exec["gather"] == I.GATHER_TILED
+    with pytest.raises(I.InfoliError):
+        net.set_exec(gather=I.GATHER_STAGED, lds_kb=40)
+    with pytest.raises(I.InfoliError):
+        net.set_fp64_degree(-1)
+    net.set_fp64_degree(0)
+    net.set_exec(gather=I.GATHER_DIRECT)
+    net.close()
+    for prec in ("fp32", "fp64"):
+        net = make_net(10, prec)
+        assert net.fp64_degree == 0
+        with pytest.raises(I.InfoliError):
+            net.set_fp64_degree(5)
+        net.set_fp64_degree(0)
+        net.close()
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "fp32k"])

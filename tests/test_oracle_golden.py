@@ -135,6 +135,43 @@ def test_gap_currents_antisymmetric_and_conserved():
     assert (O.gap_currents(O.FP64, np.full(n, -60.0), rp, col, w) == 0).all()
 
 
+def test_fp64_degree_budget_is_fp64_term_sum():
+    """The degree budget of FP32K: at or above it, I_c is ST's FP64 sum over
+    the FP32 values rounded once to float -- for float-exact voltages and
+    conductances exactly (float)gj64, which the golden ST traces pin; a
+    long-double brute force agrees to 1e-13.  Below it, the compensated FP32
+    sum, unchanged."""
+    rng = np.random.default_rng(11)
+    n = 120
+    conn = np.where(rng.random((n, n)) < rng.uniform(0.1, 0.9, (n, 1)), rng.uniform(0.02, 0.06, (n, n)), 0.0)
+    np.fill_diagonal(conn, 0.0)
+    rp, col, w = O.dense_to_csr(conn)
+    deg = np.diff(rp)
+    budget = int(np.median(deg))
+    vf = rng.uniform(-70, 10, n).astype(np.float32).astype(np.float64)
+    wf = w.astype(np.float32).astype(np.float64)
+    try:
+        O.set_fp64_degree(budget)
+        hyb = O.gap_currents(O.FP32K, vf, rp, col, wf)
+    finally:
+        O.set_fp64_degree(0)
+    plain = O.gap_currents(O.FP32K, vf, rp, col, wf)
+    ic64 = O.gap_currents(O.FP64, vf, rp, col, wf)
+    wide = deg >= budget
+    assert 0 < wide.sum() < n
+    assert np.array_equal(hyb[wide], ic64[wide].astype(np.float32).astype(np.float64))
+    assert np.array_equal(hyb[~wide], plain[~wide])
+    exact = np.zeros(n, np.longdouble)
+    for i in np.nonzero(wide)[0]:
+        for e in range(rp[i], rp[i + 1]):
+            d = np.longdouble(vf[i]) - np.longdouble(vf[col[e]])
+            exact[i] += np.longdouble(wf[e]) * (np.longdouble(0.8) * np.exp(-d * d / 100) + 0.2) * d
+    scale = float(np.max(np.abs(exact))) + 1e-300
+    assert np.max(np.abs(ic64[wide] - exact[wide].astype(np.float64))) <= 1e-13 * scale
+    # the FP64 sums sit closer to the exact value than the compensated FP32 ones
+    assert (np.abs(hyb - ic64)[wide].max() <= np.abs(plain - ic64)[wide].max())
+
+
 def test_compensated_sum_is_exact_on_cancellation():
     # receiver 0 at 0 mV; neighbours: one at -30 mV (term +B), K at -1e-4 mV
     # (K identical small terms t), one at +30 mV (term -B, the exact negative
