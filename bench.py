@@ -129,6 +129,9 @@ def main():
     ap.add_argument("--tile-cells", type=int, default=0)
     ap.add_argument("--steps-per-launch", type=int, default=0,
                     help="steps per graph launch (default 100; 500 for batched_512)")
+    ap.add_argument("--study", action="store_true",
+                    help="precision study instead of the benchmark: FP32 compensated and FP32 "
+                         "plain against FP64, max |dV| over 100 ms and spike times after it")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     args = ap.parse_args()
@@ -176,6 +179,21 @@ def main():
         net.advance(steps, trace_stride=args.stride)
         net.synchronize()
         return time.perf_counter() - t0
+
+    if args.study:
+        from infoli_amd import analysis
+        steps = args.steps if args.steps != ap.get_default("steps") else wl["steps"]
+        res = {"study": "precision", "workload": wl["name"], "cells": n, "neighbour_entries": nnz,
+               "reference": "fp64"}
+        for prec in ("fp32k", "fp32"):
+            print(f"# study {prec} vs fp64, {steps} steps", file=sys.stderr, flush=True)
+            a, b = make(prec), make("fp64")
+            res[prec] = analysis.compare_runs(a, b, steps)
+            res[prec]["exec"] = a.exec
+            a.close()
+            b.close()
+        print(json.dumps(res), flush=True)
+        return
 
     print(f"# {args.precision}: {n} cells, {nnz} neighbour entries", file=sys.stderr, flush=True)
     net = make(args.precision)
