@@ -170,6 +170,15 @@ Model<double> make_model<double>(const infoli_params &p)
     return m;
 }
 
+/* The sliced-ELL neighbour lists (sender id + conductance per entry) are
+ * larger than half of the 256 MiB Infinity Cache: they are streamed from HBM
+ * every step anyway, so their loads are marked nontemporal and leave the
+ * caches to the gathered voltages. */
+bool stream_exceeds_cache(const infoli_ctx *c)
+{
+    return (double)c->col.size() * (double)(4 + c->tsize) > 128.0 * 1024 * 1024;
+}
+
 /* some cell sums FP64 gap-junction terms (FP32 compensated, degree budget) */
 bool hyb_active(const infoli_ctx *c)
 {
@@ -191,6 +200,7 @@ StepArgs<T> make_args(const infoli_ctx *c, int64_t stride)
     a.va[0] = (T *)c->d_va[0];
     a.va[1] = (T *)c->d_va[1];
     a.deg = c->d_deg;
+    a.stream_nt = stream_exceeds_cache(c) ? 1 : 0;
     a.slice_ptr = c->d_slice_ptr;
     a.col = c->d_col;
     a.w = (const T *)c->d_w;
@@ -764,8 +774,10 @@ double tiles_per_block(const infoli_ctx *c, int R, int64_t C)
 
 /* Heuristic launch strategy (infoli_autotune measures instead):
  *  - tiled gather when the per-tile partial sums (one pair per receiver per
- *    tile) cost at most half the junction stream they replace;
- *  - else the staged gather when a 64-cell group fits the LDS budget;
+ *    tile) cost at most half the junction stream they replace, or when some
+ *    cell needs the FP64 sums of the degree budget;
+ *  - else the staged gather when a 64-cell group fits the LDS budget and
+ *    the neighbour lists fit the Infinity Cache;
  *  - else direct gathers. */
 void default_exec(infoli_ctx *c)
 {
@@ -825,7 +837,10 @@ void default_exec(infoli_ctx *c)
         c->exec.ode_split = ntiles > (double)std::max(per_cu, 1) * c->num_cus ? 2 : 1;
         return;
     }
-    if (max_block_entries_fit(c, 40)) {
+    // staged gather when a 64-cell group fits and the lists stay in the
+    // Infinity Cache; beyond it both stream from HBM and the direct gather,
+    // with nontemporal list loads, is the faster
+    if (max_block_entries_fit(c, 40) && !stream_exceeds_cache(c)) {
         c->exec.gather = GATHER_STAGED;
         c->exec.lds_kb = 40;
     }

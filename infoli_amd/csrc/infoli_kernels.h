@@ -78,6 +78,7 @@ struct StepArgs {
     const uint32_t *gl32;      // (sender << sb) | slot, sorted by sender within a block
     const uint64_t *gl64;      // (sender << 32) | slot, when the packed form does not fit
     int sb;                    // slot bits of gl32
+    int stream_nt;             // neighbour lists larger than the caches: nontemporal loads
     // 2D-tiled gap junctions (receiver block x sender chunk)
     const TileDesc *tiles;     // tiles, receiver-block major, chunks ascending
     const int32_t *rb_tile0;   // first tile of each receiver block, nrb + 1
@@ -594,7 +595,7 @@ __device__ __forceinline__ void row_terms(Acc<T, KAHAN> &acc, T vi, int d, int l
 }
 
 /* the same with the sender index loaded first (direct gather) */
-template <typename T, bool KAHAN, int L, typename IdxT, typename VOf>
+template <typename T, bool KAHAN, int L, bool NT, typename IdxT, typename VOf>
 __device__ __forceinline__ void row_terms_idx(Acc<T, KAHAN> &acc, T vi, int d, int l, int W, int64_t base,
                                               const T *__restrict__ w, const IdxT *__restrict__ idx, VOf v_of)
 {
@@ -605,8 +606,13 @@ __device__ __forceinline__ void row_terms_idx(Acc<T, KAHAN> &acc, T vi, int d, i
         for (int u = 0; u < kUnroll; u++) {
             if (kk0 + u < W) {
                 const int64_t e = base + (int64_t)(kk0 + u) * 64;
-                jv[u] = (int)idx[e];
-                wv[u] = w[e];
+                if constexpr (NT) {   // streamed once: keep it out of the caches
+                    jv[u] = (int)__builtin_nontemporal_load(&idx[e]);
+                    wv[u] = __builtin_nontemporal_load(&w[e]);
+                } else {
+                    jv[u] = (int)idx[e];
+                    wv[u] = w[e];
+                }
             }
         }
 #pragma unroll
@@ -714,8 +720,12 @@ __device__ __forceinline__ void step_block(const StepArgs<T> &a, int64_t blk, in
                                            [&](int kk) { return vj[kk * L]; });
                 } else {
                     const T *__restrict__ vsrc = (G == GATHER_VLDS) ? dyn : vin;
-                    row_terms_idx<T, KAHAN, L>(acc, vi, d, l, W, base, a.w, a.col,
-                                               [&](int j) { return vsrc[j]; });
+                    if (a.stream_nt)
+                        row_terms_idx<T, KAHAN, L, true>(acc, vi, d, l, W, base, a.w, a.col,
+                                                         [&](int j) { return vsrc[j]; });
+                    else
+                        row_terms_idx<T, KAHAN, L, false>(acc, vi, d, l, W, base, a.w, a.col,
+                                                          [&](int j) { return vsrc[j]; });
                 }
             }
         }

@@ -483,6 +483,24 @@ def test_fast_math_compartments(prec):
     assert np.abs(out - ref64).max() < 1e-3
 
 
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32k", 2e-4)])
+def test_lists_beyond_cache_direct_gather_vs_oracle(prec, tol):
+    """Neighbour lists larger than the Infinity Cache (1,000,000 cells of
+    mean degree 20: 20M entries, 160 MB in FP32) default to the direct
+    gather with nontemporal list loads; the trace matches the oracle."""
+    say(f"lists beyond cache {prec}")
+    n, steps = 1000000, 20
+    rp, col, w = I.gen_symmetric(31, n, 20.0 / (n - 1), 0.02, 0.06)
+    gcal = I.gen_uniform(31, 2, n, 0.5, 1.1)
+    iapp = I.gen_normal(31, 3, n, 1.0, 1.0)
+    net = make_net(n, prec, rp, col, w, gcal, iapp)
+    assert net.exec["gather"] == I.GATHER_DIRECT
+    out = net.advance(steps, trace_stride=5)
+    net.close()
+    ref, _ = oracle_ref(prec, n, steps, rp, col, w, gcal, iapp, stride=5)
+    assert np.abs(out - ref).max() < tol
+
+
 def test_sparse_100k_fast_math_precision():
     """FP32 + compensated sums with the fast compartment arithmetic vs FP64
     over the first 100 ms of the 100k-cell sparse network."""
