@@ -28,7 +28,14 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 GATHER_NAMES = {0: "direct", 1: "lds", 2: "staged", 3: "tiled"}
-KERNELS = {0: "k_step", 1: "k_step", 2: "k_step", 3: "k_tile_gj+k_tile_ode3"}
+TILED_KERNELS = {0: "k_tile_gj+k_tile_ode", 1: "k_tile_gj+k_tile_ode3", 2: "k_tile_gj+k_tile_dend"}
+
+
+def kernel_name(exec_cfg):
+    """The kernels of one step under this execution strategy."""
+    if exec_cfg["gather"] == 3:
+        return TILED_KERNELS[exec_cfg["ode_split"]]
+    return "k_step"
 
 
 def measured_traffic(kernel, workload, prec):
@@ -215,7 +222,7 @@ def main():
     achieved = bytes_per_launch / (mean_launch_ms * 1e-3) / 1e9
     net.close()
 
-    kname = KERNELS[exec_cfg["gather"]]
+    kname = kernel_name(exec_cfg)
     tr = measured_traffic(kname, wl["name"], args.precision)
     traffic = None if tr is None else tr * steps_per_launch
     extra = {"device_ms": round(dev_ms, 3), "exec": exec_cfg,
