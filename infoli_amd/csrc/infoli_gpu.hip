@@ -84,6 +84,7 @@ struct infoli_ctx {
     bool tiled_valid = false;
     int tiled_L = 0, tiled_kb = 0, tiled_R = 0;
     int64_t tl_ntiles = 0, tl_nrb = 0, tl_C = 0;
+    int64_t tl_nsa = 1;     // soma/axon workgroups launched beside the tiles (ode_split 2)
     TileDesc *d_tiles = nullptr;
     int32_t *d_rb_tile0 = nullptr;
     int64_t *d_tsp = nullptr;
@@ -238,6 +239,7 @@ StepArgs<T> make_args(const infoli_ctx *c, int64_t stride)
         a.tw = (const T *)c->d_tw;
         a.part = c->d_part;
         a.ntiles = (int32_t)c->tl_ntiles;
+        a.nsa = (int32_t)c->tl_nsa;
         a.fp64_deg = hyb_active(c) ? (int32_t)std::min<int64_t>(c->fp64_deg, INT32_MAX) : 0;
         a.rcells = c->tiled_R;
         a.ccells = (int32_t)c->tl_C;
@@ -360,11 +362,9 @@ int launch_step(infoli_ctx *c, const void *args, int64_t k)
         const size_t lds = (size_t)c->tl_C * c->tsize;
         allow_dyn_lds(fg, lds);
         if (c->exec.ode_split == 2) {
-            // tiles, then workgroups of block/2 somata and axons in the same
-            // launch; the dendrites (which need the tiles' partials) after
-            const int64_t per = c->exec.block / 2;
-            const int64_t nsa = (c->n + per - 1) / per;
-            HIPCHK(hipLaunchKernel(fg, dim3((unsigned)(c->tl_ntiles + nsa)), dim3(c->exec.block), kargs, lds,
+            // tiles, then the soma/axon workgroups in the same launch; the
+            // dendrites (which need the tiles' partials) after
+            HIPCHK(hipLaunchKernel(fg, dim3((unsigned)(c->tl_ntiles + c->tl_nsa)), dim3(c->exec.block), kargs, lds,
                                    c->stream));
             HIPCHK(hipLaunchKernel(k_tile_dend_tab[c->prec], dim3((unsigned)(c->npad / kBlock)), dim3(kBlock), kargs,
                                    0, c->stream));
@@ -701,6 +701,25 @@ int upload_staged(infoli_ctx *c, int kb)
     return INFOLI_OK;
 }
 
+/* Soma/axon workgroups beside the tiles (ode_split 2): as many as the
+ * workgroup slots the tiles leave free (so they hide under the tiles), but
+ * at most four cell groups each; one per group when the tiles fill every
+ * slot (they then take the tail of the last round). */
+int64_t soma_axon_workgroups(const infoli_ctx *c)
+{
+    const int64_t per = c->exec.block / 2;
+    const int64_t groups = std::max<int64_t>(1, (c->n + per - 1) / per);
+    const void *fg = hyb_active(c) ? k_tile_gj_hyb_tab[ntidx(c->exec.block)][lidx(c->exec.lanes_per_cell)]
+                                   : k_tile_gj_tab[c->prec][ntidx(c->exec.block)][lidx(c->exec.lanes_per_cell)];
+    const size_t lds = (size_t)c->tl_C * c->tsize;
+    allow_dyn_lds(fg, lds);
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fg, c->exec.block, lds) != hipSuccess) per_cu = 0;
+    const int64_t free_slots = (int64_t)per_cu * c->num_cus - c->tl_ntiles;
+    if (free_slots <= 0) return groups;
+    return std::min(groups, std::max(free_slots, (groups + 3) / 4));
+}
+
 int ensure_layout(infoli_ctx *c)
 {
     const int L = c->exec.lanes_per_cell;
@@ -721,6 +740,7 @@ int ensure_layout(infoli_ctx *c)
         RCHK(is_f32(c) ? upload_tiled<float>(c, L, c->exec.tile_cells, c->exec.lds_kb)
                        : upload_tiled<double>(c, L, c->exec.tile_cells, c->exec.lds_kb));
     }
+    if (c->exec.gather == GATHER_TILED) c->tl_nsa = soma_axon_workgroups(c);
     return INFOLI_OK;
 }
 

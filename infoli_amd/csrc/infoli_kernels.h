@@ -91,6 +91,7 @@ struct StepArgs {
     int32_t fp64_deg;          // FP32 compensated: cells with at least this many
                                // neighbours sum FP64 terms (0 = none)
     int32_t ntiles;            // tiles (workgroups past them run somata and axons)
+    int32_t nsa;               // soma/axon workgroups (each loops over cell groups)
     int32_t rcells;            // receivers per block (multiple of 64)
     int32_t ccells;            // senders per chunk
     Model<T> m;
@@ -1118,7 +1119,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_dend(StepArgs<T> a, int64_t k)
  * junctions from that chunk (L lanes per receiver, list order) into a
  * partial (sum, compensation) pair. */
 template <typename T, int L, bool KAHAN, int NT, bool HYB = false>
-__global__ __launch_bounds__(NT) void k_tile_gj(StepArgs<T> a, int64_t k)
+__global__ __launch_bounds__(NT, (sizeof(T) == 4 && !HYB) ? 8 : 1) void k_tile_gj(StepArgs<T> a, int64_t k)
 {
     extern __shared__ __align__(16) unsigned char dyn_lds[];
     T *vch = reinterpret_cast<T *>(dyn_lds);
@@ -1126,13 +1127,15 @@ __global__ __launch_bounds__(NT) void k_tile_gj(StepArgs<T> a, int64_t k)
     const int64_t t = (a.d_step ? *a.d_step : 0) + k;
     if ((int)blockIdx.x >= a.ntiles) {
         // workgroups past the tiles: somata (even waves) and axons (odd
-        // waves) of NT/2 cells, which need no gap-junction current, so they
-        // run beside the tiles
+        // waves) of groups of NT/2 cells, which need no gap-junction current,
+        // so they run beside the tiles; workgroup g takes groups g, g + nsa, ...
         const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
-        const int64_t cell = ((int64_t)(blockIdx.x - a.ntiles) * (NT / 128) + (wv >> 1)) * 64 + (threadIdx.x & 63);
-        if (cell < a.n) {
-            if (wv & 1) tile_axon_cell<T, false>(a, cell, t);
-            else tile_soma_cell<T, false>(a, cell, t);
+        for (int64_t g = (int64_t)blockIdx.x - a.ntiles; g * (NT / 2) < a.n; g += a.nsa) {
+            const int64_t cell = (g * (NT / 128) + (wv >> 1)) * 64 + (threadIdx.x & 63);
+            if (cell < a.n) {
+                if (wv & 1) tile_axon_cell<T, false>(a, cell, t);
+                else tile_soma_cell<T, false>(a, cell, t);
+            }
         }
         return;
     }
