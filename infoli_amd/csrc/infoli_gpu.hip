@@ -38,6 +38,7 @@ using namespace infoli;
 namespace {
 
 constexpr size_t kTraceRingBytes = size_t(1) << 30;   // device trace ring budget
+constexpr int64_t kTracePieceSteps = 1000;             // steps per trace piece (copy overlaps the next)
 constexpr size_t kVldsMaxBytes = 96 * 1024;           // V_dend staged in LDS up to this size
 constexpr int kDefaultLdsKb = 76;                     // staged gather: two workgroups per CU
 
@@ -121,6 +122,7 @@ struct infoli_ctx {
     GraphKey gkey;
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev_copy[2] = {nullptr, nullptr};   // trace pieces copied into staging halves
     std::vector<hipEvent_t> prof_ev;
     bool profiling = false;
     double last_ms = 0, kern_ms = 0;
@@ -962,7 +964,9 @@ extern "C" int infoli_create(infoli_t **out, const infoli_params *params, int64_
         return code;
     };
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_copy[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_copy[1], hipEventDisableTiming) != hipSuccess)
         return fail(infoli_fail(INFOLI_ERR_HIP, "infoli_create: stream/event creation failed"));
     const size_t plane = (size_t)c->npad * c->tsize;
     if (hipMalloc(&c->d_state, plane * kNV) != hipSuccess ||
@@ -1007,6 +1011,8 @@ extern "C" void infoli_destroy(infoli_t *c)
     drop_graph(c);
     for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
+    for (hipEvent_t e : c->ev_copy)
+        if (e) (void)hipEventDestroy(e);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     void *bufs[] = {c->d_state, c->d_vd[0], c->d_vd[1], c->d_vs[0], c->d_vs[1], c->d_va[0], c->d_va[1], c->d_deg, c->d_slice_ptr, c->d_col, c->d_w,
                     c->d_iapp_base, c->d_mask, c->d_tab, c->d_trace, c->d_step, c->d_bar, c->d_flag};
@@ -1416,22 +1422,36 @@ int run_steps(infoli_ctx *c, StepArgs<T> a, int64_t t, int64_t count)
     return INFOLI_OK;
 }
 
+/* Enqueue the copy of the records of steps [t, t + count) from the device
+ * ring into pinned staging buffer `buf` (of two), after the steps' kernels
+ * on the same stream; returns the number of records. */
 template <typename T>
-int copy_records(infoli_ctx *c, int64_t t, int64_t count, int64_t stride, double *trace, int64_t *rec)
+int64_t enqueue_records(infoli_ctx *c, int64_t t, int64_t count, int64_t stride, int buf, int64_t per_buf)
 {
     const int64_t first = (t + stride - 1) / stride, last = (t + count - 1) / stride;
-    if (last < first) return INFOLI_OK;
+    if (last < first) return 0;
     const int64_t nrec = last - first + 1, n = c->n;
-    RCHK(ensure_stage(c, (size_t)nrec * n * sizeof(T)));
+    T *dst = (T *)c->h_stage + (size_t)buf * per_buf * n;
     for (int64_t r = 0; r < nrec;) {
         const int64_t slot = (first + r) % c->trace_cap;
         const int64_t run = std::min(nrec - r, c->trace_cap - slot);
-        HIPCHK(hipMemcpyAsync((T *)c->h_stage + r * n, (T *)c->d_trace + slot * n, sizeof(T) * run * n,
-                              hipMemcpyDeviceToHost, c->stream));
+        if (hipMemcpyAsync(dst + r * n, (T *)c->d_trace + slot * n, sizeof(T) * run * n, hipMemcpyDeviceToHost,
+                           c->stream) != hipSuccess)
+            return infoli_fail(INFOLI_ERR_HIP, "trace copy failed");
         r += run;
     }
-    HIPCHK(hipStreamSynchronize(c->stream));
-    const T *s = (const T *)c->h_stage;
+    if (hipEventRecord(c->ev_copy[buf], c->stream) != hipSuccess) return infoli_fail(INFOLI_ERR_HIP, "event");
+    return nrec;
+}
+
+/* Wait for staging buffer `buf` and widen its records into the caller's trace. */
+template <typename T>
+int drain_records(infoli_ctx *c, int buf, int64_t per_buf, int64_t nrec, double *trace, int64_t *rec)
+{
+    if (nrec <= 0) return INFOLI_OK;
+    HIPCHK(hipEventSynchronize(c->ev_copy[buf]));
+    const int64_t n = c->n;
+    const T *s = (const T *)c->h_stage + (size_t)buf * per_buf * n;
     double *dst = trace + (*rec) * n;
     for (int64_t i = 0; i < nrec * n; i++) dst[i] = (double)s[i];
     *rec += nrec;
@@ -1444,10 +1464,14 @@ int advance_impl(infoli_ctx *c, int64_t n_steps, int64_t stride, double *trace)
     const int64_t n = c->n;
     const int64_t stride_eff = trace ? stride : 0;
     int64_t cap = 1;
+    int64_t per_piece = 1;   // records per piece: the pieces' copies and widening overlap the next piece
     if (stride_eff > 0) {
         const int64_t want = infoli_trace_records(c, n_steps, stride_eff);
         const int64_t budget = std::max<int64_t>(1, (int64_t)(kTraceRingBytes / ((size_t)n * sizeof(T))));
-        cap = std::max<int64_t>(1, std::min(want, budget));
+        per_piece = std::max<int64_t>(1, (kTracePieceSteps + stride_eff - 1) / stride_eff);
+        cap = std::max<int64_t>(1, std::min(std::min(want, per_piece), budget));
+        per_piece = cap;
+        RCHK(ensure_stage(c, (size_t)2 * cap * n * sizeof(T)));
         if (cap > c->trace_cap) {
             HIPCHK(hipStreamSynchronize(c->stream));
             if (c->d_trace) HIPCHK(hipFree(c->d_trace));
@@ -1468,16 +1492,27 @@ int advance_impl(infoli_ctx *c, int64_t n_steps, int64_t stride, double *trace)
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     int64_t t = c->step, left = n_steps, rec = 0;
-    while (left > 0) {
+    int64_t pending = 0;   // records of the previous piece still in staging buffer (piece - 1) & 1
+    for (int piece = 0; left > 0; piece++) {
         int64_t chunk = left;
-        if (stride_eff > 0) chunk = std::min<int64_t>(chunk, c->trace_cap * stride_eff);
+        if (stride_eff > 0) chunk = std::min<int64_t>(chunk, per_piece * stride_eff);
         RCHK(run_steps<T>(c, a, t, chunk));
-        if (stride_eff > 0) RCHK(copy_records<T>(c, t, chunk, stride_eff, trace, &rec));
+        if (stride_eff > 0) {
+            const int64_t got = enqueue_records<T>(c, t, chunk, stride_eff, piece & 1, per_piece);
+            if (got < 0) return (int)got;
+            // the GPU runs this piece while the previous one is widened
+            RCHK(drain_records<T>(c, (piece - 1) & 1, per_piece, pending, trace, &rec));
+            pending = got;
+        }
         t += chunk;
         left -= chunk;
         c->step = t;
     }
     HIPCHK(hipEventRecord(c->ev1, c->stream));
+    if (stride_eff > 0) {
+        const int last_buf = (int)(((n_steps + per_piece * stride_eff - 1) / (per_piece * stride_eff) - 1) & 1);
+        RCHK(drain_records<T>(c, last_buf, per_piece, pending, trace, &rec));
+    }
     hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, c->stream, c->d_flag);
     hipLaunchKernelGGL(k_check_finite<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
                        (const T *)c->d_vd[c->step & 1], (const T *)c->d_vs[c->step & 1],
