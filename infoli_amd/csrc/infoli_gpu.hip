@@ -1,10 +1,13 @@
 /*
  * infoli_gpu.hip -- C ABI runtime of the MI355X InfOli network step.
  *
- * Owns the device copies of one network (see infoli_kernels.h for the
- * layout), launches the step kernels in one of three ways (one kernel per
- * step, per-step kernels replayed as a hipGraph, or a persistent multi-step
- * kernel with a grid barrier) and moves traces back to the host.
+ * Owns the device copies of one network: state planes, the neighbour lists
+ * in the layout the chosen gather reads (sliced ELL, staged per-block
+ * tables or 2D tiles; see infoli_kernels.h), stimulus and trace ring.
+ * Chooses a launch strategy (heuristic or measured by infoli_autotune),
+ * launches the step kernels one per step, as replayed hipGraphs or as a
+ * persistent kernel with a grid barrier, and copies the trace back in
+ * pieces that overlap the following steps.
  */
 #include <hip/hip_runtime.h>
 

@@ -1,16 +1,24 @@
 /*
  * infoli_kernels.h -- device code of the InfOli network step for gfx950.
  *
- * One workgroup of 256 threads owns 256 consecutive cells.  A step is
- *   (1) gap-junction currents: L lanes share one cell's neighbour list; a
- *       wavefront covers 64/L cells per round and the workgroup runs L
- *       rounds, so all 256 cells get their I_c, which goes to LDS;
- *   (2) compartment update: one thread per cell, state in SoA planes, new
- *       V_dend written to the other half of a ping-pong buffer.
- * Neighbour lists are stored sliced-ELL: a slice is the 64/L cells one
- * wavefront handles in one round; entry p of cell c in slice s lives at
- * slice_ptr[s] + (p / L) * 64 + c * L + (p % L), so each load instruction of
- * a wavefront reads 64 consecutive entries.
+ * A step is the gap-junction currents (src/openmp/infoli.c:496-520) then the
+ * three-compartment update (src/openmp/infoli.c:548-679; FP64 twin
+ * src/single_threaded/infoli.c:371-858).  V_dend, V_soma and V_axon live in
+ * ping-pong buffers indexed by the step's parity, so every compartment reads
+ * the start-of-step voltages whichever kernel or wavefront runs it.
+ *
+ * Two families of kernels:
+ *  - fused step kernels (k_step, k_persist): a 256-thread workgroup owns a
+ *    block of cells, sums their neighbour lists (sliced ELL: entry p of cell
+ *    c in slice s at slice_ptr[s] + (p / L) * 64 + c * L + (p % L), so a load
+ *    instruction of a wavefront reads 64 consecutive entries) from V_dend in
+ *    memory, in LDS, or from a staged per-block copy, then updates the block;
+ *  - the tiled path: k_tile_gj sums the junctions of one receiver block from
+ *    one sender chunk (the chunk's V_dend in LDS, 16-bit chunk-local ids)
+ *    into per-tile partial sums, and a compartment kernel (k_tile_ode3,
+ *    k_tile_ode or k_tile_dend) merges them in chunk order and updates the
+ *    cells; somata and axons can also run in k_tile_gj beside the tiles.
+ * Each row's terms are summed in list (ascending sender) order in every path.
  */
 #pragma once
 #include <hip/hip_runtime.h>

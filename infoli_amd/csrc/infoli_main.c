@@ -27,6 +27,7 @@
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <pthread.h>
 #include <string.h>
 #include <sys/stat.h>
 #include <sys/time.h>
@@ -57,6 +58,43 @@ static void die(int rc, const char *what)
 {
     printf("Error: %s: %s\n", what, infoli_last_error()[0] ? infoli_last_error() : infoli_strerror(rc));
     exit(EXIT_FAILURE);
+}
+
+/* One chunk of trace records to format (OMP:682-696): steps [t, t + k),
+ * records at multiples of stride, nrec of them in v. */
+struct fmt_job {
+    FILE *out;
+    const double *v;
+    int64_t t, k, stride, n, nrec;
+    int gcal_mode, file_mode;
+    double pulse_amp;
+    int64_t pulse_t0, pulse_t1;
+    int bad;
+};
+
+static void *format_records(void *arg)
+{
+    struct fmt_job *j = (struct fmt_job *)arg;
+    const int64_t n = j->n;
+    int64_t r = 0;
+    for (int64_t s = j->t; s < j->t + j->k; s++) {
+        if (s % j->stride) continue;
+        const double *v = j->v + r * n;
+        if (j->gcal_mode) {
+            for (int64_t i = 0; i < n; i++) fprintf(j->out, "%.8f ", v[i]);
+        } else if (j->file_mode) {
+            fprintf(j->out, "%lld %.2f ", (long long)(s + 1), s * 0.05);
+            for (int64_t i = 0; i < n; i++) fprintf(j->out, "%.16f ", v[i]);
+        } else {
+            const double iapp = (s >= j->pulse_t0 && s < j->pulse_t1) ? j->pulse_amp : 0.0;
+            fprintf(j->out, " %lld %.2f ", (long long)(s + 1), iapp);
+            for (int64_t i = 0; i < n; i++) fprintf(j->out, "%lld : %.8f ", (long long)(i + 1), v[i]);
+        }
+        fputs("\n", j->out);
+        r++;
+    }
+    j->bad = (r != j->nrec);
+    return NULL;
 }
 
 static double now_ms(void)
@@ -240,37 +278,43 @@ int main(int argc, char **argv)
         fputs("\n", out);
     }
 
-    /* run in chunks; each chunk's records are formatted after it returns */
+    /* run in chunks into two trace buffers; a formatter thread writes chunk
+     * c's records (in step order, cells ascending) while the GPU runs chunk
+     * c + 1, and is joined before the next chunk's formatting starts */
     const int64_t max_rec = (int64_t)((64u << 20) / (sizeof(double) * (size_t)n)) + 1;
     const int64_t chunk = max_rec * stride;
-    double *trace = (double *)malloc(sizeof(double) * (size_t)max_rec * (size_t)n);
-    if (!trace) { printf("Error: Couldn't allocate the trace buffer\n"); exit(EXIT_FAILURE); }
+    double *trace[2];
+    trace[0] = (double *)malloc(sizeof(double) * (size_t)max_rec * (size_t)n);
+    trace[1] = (double *)malloc(sizeof(double) * (size_t)max_rec * (size_t)n);
+    if (!trace[0] || !trace[1]) { printf("Error: Couldn't allocate the trace buffer\n"); exit(EXIT_FAILURE); }
+    struct fmt_job jobs[2];
+    pthread_t fmt_thread;
+    int fmt_running = 0, buf = 0;
     int nonfinite = 0;
     double tic = now_ms();
-    for (int64_t t = 0; t < total_steps; t += chunk) {
+    for (int64_t t = 0; t < total_steps; t += chunk, buf ^= 1) {
         const int64_t k = total_steps - t < chunk ? total_steps - t : chunk;
         const int64_t nrec = infoli_trace_records(net, k, stride);
-        rc = infoli_advance(net, k, stride, trace);
+        rc = infoli_advance(net, k, stride, trace[buf]);
         if (rc == INFOLI_ERR_NONFINITE) nonfinite = 1;
         else if (rc) die(rc, "advance");
-        int64_t r = 0;
-        for (int64_t s = t; s < t + k; s++) {
-            if (s % stride) continue;
-            const double *v = trace + r * n;
-            if (gcal_file) {
-                for (int64_t i = 0; i < n; i++) fprintf(out, "%.8f ", v[i]);
-            } else if (in_file) {
-                fprintf(out, "%lld %.2f ", (long long)(s + 1), s * 0.05);
-                for (int64_t i = 0; i < n; i++) fprintf(out, "%.16f ", v[i]);
-            } else {
-                const double iapp = (s >= pulse_t0 && s < pulse_t1) ? pulse_amp : 0.0;
-                fprintf(out, " %lld %.2f ", (long long)(s + 1), iapp);
-                for (int64_t i = 0; i < n; i++) fprintf(out, "%lld : %.8f ", (long long)(i + 1), v[i]);
-            }
-            fputs("\n", out);
-            r++;
+        if (fmt_running) {
+            pthread_join(fmt_thread, NULL);
+            fmt_running = 0;
+            if (jobs[buf ^ 1].bad) { printf("Error: trace record count mismatch\n"); exit(EXIT_FAILURE); }
         }
-        if (r != nrec) { printf("Error: trace record count mismatch\n"); exit(EXIT_FAILURE); }
+        jobs[buf] = (struct fmt_job){out, trace[buf], t, k, stride, n, nrec, gcal_file != NULL, in_file != NULL,
+                                     pulse_amp, pulse_t0, pulse_t1, 0};
+        if (pthread_create(&fmt_thread, NULL, format_records, &jobs[buf]) == 0) {
+            fmt_running = 1;
+        } else {
+            format_records(&jobs[buf]);   /* no thread: format here */
+            if (jobs[buf].bad) { printf("Error: trace record count mismatch\n"); exit(EXIT_FAILURE); }
+        }
+    }
+    if (fmt_running) {
+        pthread_join(fmt_thread, NULL);
+        if (jobs[buf ^ 1].bad) { printf("Error: trace record count mismatch\n"); exit(EXIT_FAILURE); }
     }
     double toc = now_ms();
     fclose(out);
@@ -284,7 +328,8 @@ int main(int argc, char **argv)
             fclose(out);
         }
     }
-    free(trace);
+    free(trace[0]);
+    free(trace[1]);
     infoli_destroy(net);
     return 0;
 }
