@@ -166,7 +166,35 @@ int infoli_set_iapp_table(infoli_t *ctx, int64_t t_first, int64_t n_steps,
 int infoli_set_state(infoli_t *ctx, int var, const double *values);
 int infoli_get_state(infoli_t *ctx, int var, double *values);
 
-/* ---- the step ----------------------------------------------------------- */
+/* Write every cell's state in printState's layout
+ * (src/single_threaded/infoli.c:971-1005): one line per cell, 16 values
+ * "%.8lf " in the order V_soma, Sodium_m, Potassium_n, Potassium_x_s,
+ * Calcium_k, Calcium_l, V_dend, Calcium_r, Potassium_s, Hcurrent_q,
+ * Ca2Plus, I_CaH, V_axon, Sodium_m_a, Sodium_h_a, Potassium_x_a.  Replaces
+ * the PRINTSTATE dump (src/single_threaded/infoli.c:343-347). */
+int infoli_write_state(infoli_t *ctx, const char *path);
+
+/* Checkpoint and resume: all 19 per-cell variables in FP64 and the step
+ * count (so a resumed run continues the pulse window and the Iapp table
+ * where it stopped), in a binary file.  Loading needs the same cell count;
+ * it converts between precisions.  No reference counterpart: printState
+ * drops Sodium_h, Potassium_p and g_CaL and rounds to 8 decimals. */
+int infoli_save_checkpoint(infoli_t *ctx, const char *path);
+int infoli_load_checkpoint(infoli_t *ctx, const char *path);
+
+/* Advance every cell alone, with no gap-junction current and no applied
+ * current, for its own number of steps (steps[i] >= 0): the closed-circuit
+ * warm-up of RAND_INIT (src/single_threaded/infoli.c:219-236, CompDend with
+ * randomness 1).  The step count does not change. */
+int infoli_isolated_steps(infoli_t *ctx, const int64_t *steps);
+
+/* RAND_INIT's randomness from the counter generator (stream 7):
+ * g_cal[i] = 0.6 + 0.2*(k % 100)/100 (src/single_threaded/infoli.c:212-214)
+ * and steps[i] = (u % ceil(100/delta)) | 1 (src/single_threaded/infoli.c:221-222),
+ * k, u the cell's two 64-bit draws.  Host only. */
+int infoli_gen_rand_init(uint64_t seed, int64_t n, double delta, double *g_cal, int64_t *steps);
+
+/* ---- the step ----------------------------------------------------------- *//* ---- the step ----------------------------------------------------------- */
 
 /* Advance n_steps timesteps.  After step t (0-based, counted from context
  * creation) with t % trace_stride == 0, V_axon of every cell is appended to

@@ -25,6 +25,7 @@ from .network import (  # noqa: F401
     gen_directed,
     gen_mask,
     gen_normal,
+    gen_rand_init,
     gen_symmetric,
     gen_uniform,
     library,

@@ -1220,6 +1220,122 @@ extern "C" int infoli_get_state(infoli_t *c, int var, double *values)
 }
 
 /* ------------------------------------------------------------------------ */
+/* state files and the isolated warm-up                                      */
+/* ------------------------------------------------------------------------ */
+
+extern "C" int infoli_write_state(infoli_t *c, const char *path)
+{
+    infoli_clear_error();
+    RCHK(validate_ctx(c, "infoli_write_state"));
+    if (!path) return infoli_fail(INFOLI_ERR_ARG, "infoli_write_state: null path");
+    // printState's column order (src/single_threaded/infoli.c:977-992)
+    static const int cols[16] = {INFOLI_V_SOMA,      INFOLI_SODIUM_M,  INFOLI_POTASSIUM_N,  INFOLI_POTASSIUM_X_S,
+                                 INFOLI_CALCIUM_K,   INFOLI_CALCIUM_L, INFOLI_V_DEND,       INFOLI_CALCIUM_R,
+                                 INFOLI_POTASSIUM_S, INFOLI_HCURRENT_Q, INFOLI_CA2PLUS,     INFOLI_I_CAH,
+                                 INFOLI_V_AXON,      INFOLI_SODIUM_M_A, INFOLI_SODIUM_H_A,  INFOLI_POTASSIUM_X_A};
+    std::vector<double> v((size_t)16 * c->n);
+    for (int j = 0; j < 16; j++) RCHK(infoli_get_state(c, cols[j], v.data() + (size_t)j * c->n));
+    FILE *fd = fopen(path, "w");
+    if (!fd) return infoli_fail(INFOLI_ERR_IO, "infoli_write_state: cannot create %s", path);
+    for (int64_t i = 0; i < c->n; i++) {
+        for (int j = 0; j < 16; j++) fprintf(fd, "%.8lf ", v[(size_t)j * c->n + i]);
+        fprintf(fd, "\n");
+    }
+    if (fclose(fd) != 0) return infoli_fail(INFOLI_ERR_IO, "infoli_write_state: write to %s failed", path);
+    return INFOLI_OK;
+}
+
+namespace {
+struct CheckpointHeader {
+    char magic[8];        // "INFOLICK"
+    int32_t version;      // 1
+    int32_t precision;    // INFOLI_FP64 / FP32 / FP32_KAHAN of the writer
+    int64_t n_cells;
+    int64_t step;
+    int32_t n_vars;       // INFOLI_NVARS
+    int32_t reserved;
+};
+}  // namespace
+
+extern "C" int infoli_save_checkpoint(infoli_t *c, const char *path)
+{
+    infoli_clear_error();
+    RCHK(validate_ctx(c, "infoli_save_checkpoint"));
+    if (!path) return infoli_fail(INFOLI_ERR_ARG, "infoli_save_checkpoint: null path");
+    std::vector<double> v((size_t)INFOLI_NVARS * c->n);
+    for (int j = 0; j < INFOLI_NVARS; j++) RCHK(infoli_get_state(c, j, v.data() + (size_t)j * c->n));
+    CheckpointHeader h{};
+    memcpy(h.magic, "INFOLICK", 8);
+    h.version = 1;
+    h.precision = c->prec;
+    h.n_cells = c->n;
+    h.step = c->step;
+    h.n_vars = INFOLI_NVARS;
+    FILE *fd = fopen(path, "wb");
+    if (!fd) return infoli_fail(INFOLI_ERR_IO, "infoli_save_checkpoint: cannot create %s", path);
+    const bool ok = fwrite(&h, sizeof h, 1, fd) == 1 && fwrite(v.data(), sizeof(double), v.size(), fd) == v.size();
+    if (fclose(fd) != 0 || !ok) return infoli_fail(INFOLI_ERR_IO, "infoli_save_checkpoint: write to %s failed", path);
+    return INFOLI_OK;
+}
+
+extern "C" int infoli_load_checkpoint(infoli_t *c, const char *path)
+{
+    infoli_clear_error();
+    RCHK(validate_ctx(c, "infoli_load_checkpoint"));
+    if (!path) return infoli_fail(INFOLI_ERR_ARG, "infoli_load_checkpoint: null path");
+    FILE *fd = fopen(path, "rb");
+    if (!fd) return infoli_fail(INFOLI_ERR_IO, "infoli_load_checkpoint: cannot open %s", path);
+    CheckpointHeader h{};
+    std::vector<double> v;
+    bool ok = fread(&h, sizeof h, 1, fd) == 1 && memcmp(h.magic, "INFOLICK", 8) == 0 && h.version == 1 &&
+              h.n_vars == INFOLI_NVARS;
+    if (ok && h.n_cells != c->n) {
+        fclose(fd);
+        return infoli_fail(INFOLI_ERR_ARG, "infoli_load_checkpoint: %s holds %lld cells, the network has %lld", path,
+                           (long long)h.n_cells, (long long)c->n);
+    }
+    if (ok) {
+        v.resize((size_t)INFOLI_NVARS * c->n);
+        ok = fread(v.data(), sizeof(double), v.size(), fd) == v.size() && h.step >= 0;
+    }
+    fclose(fd);
+    if (!ok) return infoli_fail(INFOLI_ERR_IO, "infoli_load_checkpoint: %s is not an infoli checkpoint", path);
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->step = h.step;   // first: the voltages go to the buffers of this step's parity
+    for (int j = 0; j < INFOLI_NVARS; j++) RCHK(infoli_set_state(c, j, v.data() + (size_t)j * c->n));
+    return INFOLI_OK;
+}
+
+extern "C" int infoli_isolated_steps(infoli_t *c, const int64_t *steps)
+{
+    infoli_clear_error();
+    RCHK(validate_ctx(c, "infoli_isolated_steps"));
+    if (!steps) return infoli_fail(INFOLI_ERR_ARG, "infoli_isolated_steps: null steps");
+    for (int64_t i = 0; i < c->n; i++)
+        if (steps[i] < 0) return infoli_fail(INFOLI_ERR_ARG, "infoli_isolated_steps: steps[%lld] < 0", (long long)i);
+    HIPCHK(hipSetDevice(c->device));
+    int64_t *d_steps = nullptr;
+    HIPCHK(hipMalloc(&d_steps, sizeof(int64_t) * c->n));
+    int rc = INFOLI_OK;
+    if (hipMemcpy(d_steps, steps, sizeof(int64_t) * c->n, hipMemcpyHostToDevice) != hipSuccess) {
+        rc = infoli_fail(INFOLI_ERR_HIP, "infoli_isolated_steps: copy failed");
+    } else {
+        const dim3 grid((unsigned)((c->n + kBlock - 1) / kBlock));
+        if (is_f32(c))
+            hipLaunchKernelGGL(k_isolated<float>, grid, dim3(kBlock), 0, c->stream, make_args<float>(c, 0), c->step,
+                               (const int64_t *)d_steps);
+        else
+            hipLaunchKernelGGL(k_isolated<double>, grid, dim3(kBlock), 0, c->stream, make_args<double>(c, 0), c->step,
+                               (const int64_t *)d_steps);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
+            rc = infoli_fail(INFOLI_ERR_HIP, "infoli_isolated_steps: kernel failed");
+    }
+    (void)hipFree(d_steps);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
 /* execution                                                                 */
 /* ------------------------------------------------------------------------ */
 

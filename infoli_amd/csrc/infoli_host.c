@@ -296,7 +296,26 @@ int infoli_gen_mask(uint64_t seed, int stream, int64_t n, double frac, uint8_t *
     return INFOLI_OK;
 }
 
+/* RAND_INIT's per-cell randomness (src/single_threaded/infoli.c:212-222)
+ * from the counter generator, stream 7: g_CaL = 0.6 + 0.2*(k % 100)/100
+ * and warm-up steps (u % ceil(100/DELTA)) | 1, k and u the two 64-bit
+ * draws of cell i. */
+int infoli_gen_rand_init(uint64_t seed, int64_t n, double delta, double *g_cal, int64_t *steps)
+{
+    if (n < 0 || (n > 0 && (!g_cal || !steps)) || !(delta > 0))
+        return infoli_fail(INFOLI_ERR_ARG, "infoli_gen_rand_init: bad n/delta/outputs");
+    const uint64_t span = (uint64_t)ceil(100 / delta);
+    for (int64_t i = 0; i < n; i++) {
+        const uint64_t k = infoli_rng_bits(seed, 7, (uint64_t)i, 0);
+        const uint64_t u = infoli_rng_bits(seed, 7, (uint64_t)i, 1);
+        g_cal[i] = 0.6 + (0.2 * (double)(k % 100)) / 100;
+        steps[i] = (int64_t)((u % span) | 1u);
+    }
+    return INFOLI_OK;
+}
+
 /* ------------------------------------------------------------------------ */
+/* files                                                                     *//* ------------------------------------------------------------------------ */
 /* files                                                                     */
 /* ------------------------------------------------------------------------ */
 

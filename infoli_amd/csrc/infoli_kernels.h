@@ -1216,6 +1216,23 @@ __global__ __launch_bounds__(NT, (sizeof(T) == 4 && !HYB) ? 8 : 1) void k_tile_g
     }
 }
 
+/* Every cell alone for steps[cell] steps: no gap-junction current, no
+ * applied current (the closed-circuit warm-up of
+ * src/single_threaded/infoli.c:219-236, CompDend with randomness 1).  The
+ * state stays in the buffers of step t (the warm-up precedes step t). */
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_isolated(StepArgs<T> a, int64_t t, const int64_t *steps)
+{
+    const int64_t cell = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (cell >= a.n) return;
+    const int64_t s = steps[cell];
+    if (s <= 0) return;
+    typename CellOf<T>::type c;
+    load_cell(a, cell, t, c);
+    for (int64_t k = 0; k < s; k++) ode(a.m, c, T(0), T(0));
+    store_cell(a, cell, t + 1, c);   // step t + 1's "new" buffers are step t's
+}
+
 __global__ void k_reset_barrier(GridBarrier *gb)
 {
     gb->count = 0;

@@ -52,6 +52,8 @@ static void print_usage(void)
     printf("GPU build options: -g <gcal file> -m <conn matrix> -r <conn triplets> -x fp32|fp32k|fp64\n");
     printf("                   -s <seed> -k <trace stride> -D <device> -a (autotune)\n");
     printf("                   -b <degree> (fp32k: FP64 sums for cells of at least this degree; 0 = none)\n");
+    printf("                   -R <seed> (RAND_INIT: random g_CaL and isolated warm-up) -S <state dump file>\n");
+    printf("                   -L <checkpoint to resume from> -C <checkpoint to write at the end>\n");
 }
 
 static void die(int rc, const char *what)
@@ -116,6 +118,9 @@ int main(int argc, char **argv)
     int64_t stride = 100;
     int stride_given = 0;
     int64_t fp64_degree = -1;   /* -1: the library's default */
+    int rand_init = 0;
+    uint64_t rand_seed = 0;
+    const char *state_file = NULL, *ck_in = NULL, *ck_out = NULL;
 
     /* config next to the executable (OMP:113-121) */
     ssize_t len = readlink("/proc/self/exe", params_file, PATH_MAX);
@@ -129,7 +134,7 @@ int main(int argc, char **argv)
     strcat(params_file, "default.conf");
 
     int opt;
-    while ((opt = getopt(argc, argv, "hn:p:t:i:c:o:g:m:r:x:s:k:D:ab:")) != -1) {
+    while ((opt = getopt(argc, argv, "hn:p:t:i:c:o:g:m:r:x:s:k:D:ab:R:S:L:C:")) != -1) {
         switch (opt) {
         case 'h': print_usage(); return 0;
         case 'n':
@@ -185,6 +190,10 @@ int main(int argc, char **argv)
             fp64_degree = strtoll(optarg, NULL, 10);
             if (fp64_degree < 0) { print_usage(); exit(EXIT_FAILURE); }
             break;
+        case 'R': rand_init = 1; rand_seed = strtoull(optarg, NULL, 10); break;
+        case 'S': state_file = optarg; break;
+        case 'L': ck_in = optarg; break;
+        case 'C': ck_out = optarg; break;
         default: print_usage(); exit(EXIT_FAILURE);
         }
     }
@@ -244,6 +253,24 @@ int main(int argc, char **argv)
         free(g);
     }
 
+    if (rand_init) {
+        /* RAND_INIT (src/single_threaded/infoli.c:212-236): random g_CaL, then
+         * every cell alone for a random odd number of steps */
+        double *g = (double *)malloc(sizeof(double) * (size_t)n);
+        int64_t *ws = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+        if (!g || !ws) { printf("Error: Couldn't allocate memory for channels\n"); exit(EXIT_FAILURE); }
+        rc = infoli_gen_rand_init(rand_seed, n, params.v[INFOLI_P_DELTA], g, ws);
+        if (!rc) rc = infoli_set_gcal(net, g);
+        if (!rc) rc = infoli_isolated_steps(net, ws);
+        if (rc) die(rc, "random initialisation");
+        free(g);
+        free(ws);
+    }
+    if (ck_in) {
+        rc = infoli_load_checkpoint(net, ck_in);
+        if (rc) die(rc, "checkpoint");
+    }
+
     int64_t total_steps = (int64_t)(sim_time / params.v[INFOLI_P_DELTA]);  /* OMP:478 */
     double pulse_amp = 6;
     int64_t pulse_t0 = 20000, pulse_t1 = 20500 - 1;                        /* OMP:482-485 */
@@ -292,8 +319,9 @@ int main(int argc, char **argv)
     int fmt_running = 0, buf = 0;
     int nonfinite = 0;
     double tic = now_ms();
-    for (int64_t t = 0; t < total_steps; t += chunk, buf ^= 1) {
-        const int64_t k = total_steps - t < chunk ? total_steps - t : chunk;
+    const int64_t start = infoli_step_count(net);   /* > 0 when resumed from a checkpoint */
+    for (int64_t t = start; t < start + total_steps; t += chunk, buf ^= 1) {
+        const int64_t k = start + total_steps - t < chunk ? start + total_steps - t : chunk;
         const int64_t nrec = infoli_trace_records(net, k, stride);
         rc = infoli_advance(net, k, stride, trace[buf]);
         if (rc == INFOLI_ERR_NONFINITE) nonfinite = 1;
@@ -327,6 +355,14 @@ int main(int argc, char **argv)
             fprintf(out, "Execution Time for Simulation in ms: %0.2f\n", toc - tic);
             fclose(out);
         }
+    }
+    if (state_file) {
+        rc = infoli_write_state(net, state_file);
+        if (rc) die(rc, "state dump");
+    }
+    if (ck_out) {
+        rc = infoli_save_checkpoint(net, ck_out);
+        if (rc) die(rc, "checkpoint");
     }
     free(trace[0]);
     free(trace[1]);

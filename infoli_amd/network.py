@@ -74,6 +74,11 @@ def library():
         "infoli_set_connectivity_csr": ([P, P, P, P], i32),
         "infoli_set_partition": ([P, i32, P], i32),
         "infoli_set_fp64_degree": ([P, i64], i32),
+        "infoli_write_state": ([P, ctypes.c_char_p], i32),
+        "infoli_save_checkpoint": ([P, ctypes.c_char_p], i32),
+        "infoli_load_checkpoint": ([P, ctypes.c_char_p], i32),
+        "infoli_isolated_steps": ([P, P], i32),
+        "infoli_gen_rand_init": ([u64, i64, d, P, P], i32),
         "infoli_get_fp64_degree": ([P], i64),
         "infoli_set_gcal": ([P, P], i32),
         "infoli_set_iapp": ([P, P], i32),
@@ -217,6 +222,25 @@ class Network:
         offsets = np.ascontiguousarray(offsets, np.int64)
         _check(library().infoli_set_partition(self._h, len(offsets) - 1, _p(offsets)),
                "set_partition")
+
+    def write_state(self, path):
+        """printState's 16-column dump of every cell (%.8lf)."""
+        _check(library().infoli_write_state(self._h, os.fsencode(path)), "write_state")
+
+    def save_checkpoint(self, path):
+        """All 19 variables (FP64) and the step count, binary."""
+        _check(library().infoli_save_checkpoint(self._h, os.fsencode(path)), "save_checkpoint")
+
+    def load_checkpoint(self, path):
+        _check(library().infoli_load_checkpoint(self._h, os.fsencode(path)), "load_checkpoint")
+
+    def isolated_steps(self, steps):
+        """Every cell alone (no gap-junction or applied current) for steps[i]
+        steps: RAND_INIT's warm-up.  The step count does not change."""
+        steps = np.ascontiguousarray(steps, np.int64)
+        if steps.shape != (self.n,):
+            raise ValueError(f"steps must have {self.n} entries")
+        _check(library().infoli_isolated_steps(self._h, _p(steps)), "isolated_steps")
 
     def set_fp64_degree(self, min_degree):
         """FP32 compensated only: cells with at least `min_degree` neighbours
@@ -364,6 +388,14 @@ def gen_uniform(seed, stream, n, lo, hi):
     _check(library().infoli_gen_uniform(int(seed), int(stream), int(n), float(lo), float(hi),
                                         _p(out)))
     return out
+
+
+def gen_rand_init(seed, n, delta=0.05):
+    """RAND_INIT's per-cell g_CaL and warm-up step counts, seeded."""
+    g = np.zeros(n, np.float64)
+    st = np.zeros(n, np.int64)
+    _check(library().infoli_gen_rand_init(int(seed), int(n), float(delta), _p(g), _p(st)), "gen_rand_init")
+    return g, st
 
 
 def gen_normal(seed, stream, n, mean, sd):

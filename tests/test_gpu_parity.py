@@ -501,6 +501,98 @@ def test_lists_beyond_cache_direct_gather_vs_oracle(prec, tol):
     assert np.abs(out - ref).max() < tol
 
 
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-9), ("fp32k", 1e-4)])
+def test_isolated_steps_rand_init_vs_oracle(prec, tol):
+    """RAND_INIT's warm-up: every cell alone (no gap-junction current, no
+    Iapp) for its own odd number of steps, on a coupled network (the coupling
+    must not act); each cell equals the oracle's single-cell run."""
+    say(f"isolated steps {prec}")
+    n = 40
+    rp, col, w, _, _ = _random_case(n, p=0.3, seed=13)
+    gcal, steps = I.gen_rand_init(5, n, 0.05)
+    steps[0] = 0                           # a cell that does not move
+    net = make_net(n, prec, rp, col, w, gcal)
+    before = np.stack([net.get_state(v) for v in I.VARS])
+    net.isolated_steps(steps)
+    assert net.step_count == 0
+    after = np.stack([net.get_state(v) for v in I.VARS])
+    net.close()
+    assert np.array_equal(after[:, 0], before[:, 0])
+    O.set_fp64_degree(0)
+    for i in range(1, n):
+        st = O.initial_state(1)
+        st[O.G_CAL] = gcal[i]
+        O.run(PRECS[prec], st, int(steps[i]))
+        for v in range(len(I.VARS)):
+            ref = st[v][0]
+            assert abs(after[v, i] - ref) <= tol * max(1.0, abs(ref)), (i, I.VARS[v], after[v, i], ref)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32k"])
+@pytest.mark.parametrize("split_at", [150, 151])
+def test_checkpoint_resume_bitwise(prec, split_at, tmp_path):
+    """Save after `split_at` steps, continue; a fresh network loading the
+    checkpoint continues identically (trace and all 19 variables), whichever
+    parity the step count has."""
+    say(f"checkpoint {prec} at {split_at}")
+    n = 500
+    rp, col, w, gcal, iapp = _random_case(n, p=0.05, seed=21)
+    mask = I.gen_mask(21, 4, n, 0.4)
+    ck = str(tmp_path / "state.ck")
+
+    def build():
+        net = make_net(n, prec, rp, col, w, gcal, iapp)
+        net.set_iapp_pulse(3.0, 100, 260, mask)
+        return net
+
+    a = build()
+    a.advance(split_at, trace_stride=1)
+    a.save_checkpoint(ck)
+    ta = a.advance(200, trace_stride=7)
+    sa = np.stack([a.get_state(v) for v in I.VARS])
+    a.close()
+    b = build()
+    b.load_checkpoint(ck)
+    assert b.step_count == split_at
+    tb = b.advance(200, trace_stride=7)
+    sb = np.stack([b.get_state(v) for v in I.VARS])
+    b.close()
+    assert np.array_equal(ta, tb)
+    assert np.array_equal(sa, sb)
+    c = make_net(n + 1, prec)
+    with pytest.raises(I.InfoliError):
+        c.load_checkpoint(ck)
+    bad = tmp_path / "bad.ck"
+    bad.write_bytes(b"not a checkpoint")
+    with pytest.raises(I.InfoliError):
+        c.load_checkpoint(str(bad))
+    c.close()
+
+
+def test_write_state_printstate_layout(tmp_path):
+    """printState's dump: 16 columns per cell in its order, "%.8lf "."""
+    say("write_state layout")
+    n = 30
+    rp, col, w, gcal, iapp = _random_case(n, p=0.2, seed=2)
+    net = make_net(n, "fp64", rp, col, w, gcal, iapp)
+    net.advance(77)
+    path = tmp_path / "lastStateDump.txt"
+    net.write_state(str(path))
+    order = ["V_soma", "Sodium_m", "Potassium_n", "Potassium_x_s", "Calcium_k", "Calcium_l", "V_dend",
+             "Calcium_r", "Potassium_s", "Hcurrent_q", "Ca2Plus", "I_CaH", "V_axon", "Sodium_m_a",
+             "Sodium_h_a", "Potassium_x_a"]
+    vals = {v: net.get_state(v) for v in order}
+    net.close()
+    lines = path.read_text().split("\n")
+    assert lines[-1] == "" and len(lines) == n + 1
+    for i, line in enumerate(lines[:-1]):
+        assert line.endswith(" ")
+        fields = line.split()
+        assert len(fields) == 16
+        for j, v in enumerate(order):
+            assert fields[j] == "%.8f" % vals[v][i]
+
+
 def test_sparse_100k_fast_math_precision():
     """FP32 + compensated sums with the fast compartment arithmetic vs FP64
     over the first 100 ms of the 100k-cell sparse network."""
@@ -549,6 +641,40 @@ def test_driver_reproduces_regression_baseline(golden):
                     fh.write("".join("%.8f " % x for x in row) + "\n")
             r = subprocess.run([checker, "10"], cwd=d, capture_output=True, text=True)
             assert "Check was passed successfully" in r.stdout, r.stdout
+
+
+def test_driver_checkpoint_resume_and_rand_init():
+    """Driver: a 20 ms run resumed from the checkpoint of a first 20 ms run
+    ends in the state of one 40 ms run (printState dumps compared as text);
+    -R's warm-up equals the library's isolated steps with the same seed."""
+    say("driver checkpoint / RAND_INIT")
+    exe = os.path.join(O.ROOT, "infoli_amd", "bin", "infoli.x")
+    base = [exe, "-n", "20", "-p", "0.3", "-s", "3", "-x", "fp64", "-c", os.path.join(O.GOLDEN, "default.conf")]
+    with tempfile.TemporaryDirectory() as d:
+        def run(*extra):
+            r = subprocess.run(base + list(extra), cwd=d, capture_output=True, text=True, timeout=120)
+            assert r.returncode == 0, r.stdout + r.stderr
+        run("-t", "20", "-C", "ck1", "-S", "st1")
+        run("-t", "20", "-L", "ck1", "-S", "st2", "-o", "out2.txt")
+        run("-t", "40", "-S", "st3")
+        st2, st3 = open(os.path.join(d, "st2")).read(), open(os.path.join(d, "st3")).read()
+        assert st2 == st3 and st2 != open(os.path.join(d, "st1")).read()
+        # the resumed trace continues the step numbering: records at steps 401, 501, ...
+        rec = [ln.split()[0] for ln in open(os.path.join(d, "out2.txt")).read().splitlines()
+               if ln.strip() and not ln.startswith(("#", "Execution"))]
+        assert rec == ["401", "501", "601", "701"]
+        run("-t", "1", "-R", "5", "-S", "st4")     # warm-up, then 20 coupled steps
+        n = 20
+        gcal, steps = I.gen_rand_init(5, n, 0.05)
+        rp, col, w = I.gen_directed(3, n, 0.3, 0.04)
+        net = I.Network(n, precision="fp64")
+        net.set_connectivity(rp, col, w)
+        net.set_gcal(gcal)
+        net.isolated_steps(steps)
+        net.advance(20)
+        net.write_state(os.path.join(d, "st5"))
+        net.close()
+        assert open(os.path.join(d, "st4")).read() == open(os.path.join(d, "st5")).read()
 
 
 def test_driver_reference_format():

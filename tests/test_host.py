@@ -201,6 +201,22 @@ def test_gen_values():
     assert (I.gen_uniform(3, 2, 100, 0.5, 1.1) == u[:100]).all()
 
 
+def test_gen_rand_init_equals_definition():
+    """RAND_INIT's g_CaL 0.6 + 0.2*(k % 100)/100 and odd warm-up step counts
+    below ceil(100/DELTA) (src/single_threaded/infoli.c:212-222), k and u the
+    cell's draws of stream 7."""
+    n, seed = 500, 99
+    g, st = I.gen_rand_init(seed, n, 0.05)
+    for i in range(n):
+        k, u = _bits(seed, 7, i, 0), _bits(seed, 7, i, 1)
+        assert g[i] == 0.6 + (0.2 * (k % 100)) / 100
+        assert st[i] == (u % 2000) | 1
+    assert (st % 2 == 1).all() and st.min() >= 1 and st.max() < 2000
+    assert g.min() >= 0.6 and g.max() <= 0.6 + 0.2 * 99 / 100 + 1e-15
+    g2, st2 = I.gen_rand_init(seed + 1, n, 0.05)
+    assert not np.array_equal(st, st2)
+
+
 def test_gen_argument_errors():
     with pytest.raises(I.InfoliError):
         I.gen_symmetric(1, 0, 0.1, 0.04)
