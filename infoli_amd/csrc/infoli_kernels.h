@@ -1000,7 +1000,7 @@ __device__ __forceinline__ T tiled_ic(const StepArgs<T> &a, int64_t cell)
     const int64_t rb = cell / a.rcells, rloc = cell - rb * a.rcells;
     const int32_t t0 = a.rb_tile0[rb], t1 = a.rb_tile0[rb + 1];   // uniform per wavefront
     Acc<T, KAHAN> acc;
-    constexpr int B = 8;
+    constexpr int B = 12;   // partials in flight: one batch for up to 12 sender chunks
     for (int32_t tt = t0; tt < t1; tt += B) {
         P v[B];
 #pragma unroll
@@ -1156,14 +1156,13 @@ __global__ __launch_bounds__(NT, (sizeof(T) == 4 && !HYB) ? 8 : 1) void k_tile_g
     const int cw = lane / L, l = lane % L;
     const int64_t s0 = (int64_t)td.ch * a.ccells;
     const int64_t cn = (a.n - s0 < a.ccells) ? a.n - s0 : a.ccells;
-    tile_chunk_to_lds<T, NT>(vin + s0, vch, cn, tid);
-    __syncthreads();
     const int64_t cellb = (int64_t)td.rb * a.rcells;
     constexpr int NW = NT / 64;
     const int loff = cw * L + l;
     // wavefront w takes slices w, 2NW-1-w, 2NW+w, ...: the tile's slices are
     // ordered by width, so the snake evens out the waves' work.  The next
-    // slice's receiver entries are loaded while the current one is summed.
+    // slice's receiver entries are loaded while the current one is summed;
+    // the first slice's while the chunk is copied.
     const auto snake = [&](int r) { return r * NW + ((r & 1) ? NW - 1 - wave : wave); };
     int sl = snake(0), rloc = 0, d = 0;
     if (sl < td.nsl) {
@@ -1171,6 +1170,8 @@ __global__ __launch_bounds__(NT, (sizeof(T) == 4 && !HYB) ? 8 : 1) void k_tile_g
         rloc = a.rlist[slot];
         d = a.rdeg[slot];
     }
+    tile_chunk_to_lds<T, NT>(vin + s0, vch, cn, tid);
+    __syncthreads();
     for (int r = 0; r * NW < td.nsl; r++) {
         const int sn = snake(r + 1);
         int rloc_n = 0, d_n = 0;
