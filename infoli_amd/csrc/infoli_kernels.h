@@ -24,6 +24,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace infoli {
 
 constexpr int kBlock = 256;          // threads per workgroup = cells per workgroup
@@ -937,25 +939,28 @@ __device__ __forceinline__ void tile_row_terms(Acc<T, KAHAN> &acc, double &accd,
 {
     const uint32_t oi = (uint32_t)loff * (uint32_t)sizeof(uint16_t);
     const uint32_t ow = (uint32_t)loff * (uint32_t)sizeof(T);
-    for (int kk0 = 0; kk0 < W; kk0 += kUnroll) {
+    // one batch of kUnroll slots from kk0; FULL: all of them inside the
+    // slice width, so no slot needs a width check
+    const auto batch = [&](auto full, int kk0) {
+        constexpr bool FULL = decltype(full)::value;
         const uint16_t *ib = irow + (int64_t)kk0 * 64;
         const T *wb = wrow + (int64_t)kk0 * 64;
         int jv[kUnroll];
         T wv[kUnroll], vv[kUnroll];
 #pragma unroll
         for (int u = 0; u < kUnroll; u++) {
-            if (kk0 + u < W) {
+            if (FULL || kk0 + u < W) {
                 jv[u] = (int)ld_off(ib + u * 64, oi);
                 if constexpr (!WC) wv[u] = ld_off(wb + u * 64, ow);
             }
         }
 #pragma unroll
         for (int u = 0; u < kUnroll; u++)
-            if (kk0 + u < W) vv[u] = vch[jv[u]];
+            if (FULL || kk0 + u < W) vv[u] = vch[jv[u]];
         if (HYB && wide) {
 #pragma unroll
             for (int u = 0; u < kUnroll; u++) {
-                if (kk0 + u < W) {
+                if (FULL || kk0 + u < W) {
                     const double term = gj_term((double)vi, (double)vv[u], (double)(WC ? wc : wv[u]));
                     accd += ((kk0 + u) * L + l < d) ? term : 0.0;
                 }
@@ -963,13 +968,17 @@ __device__ __forceinline__ void tile_row_terms(Acc<T, KAHAN> &acc, double &accd,
         } else {
 #pragma unroll
             for (int u = 0; u < kUnroll; u++) {
-                if (kk0 + u < W) {
+                if (FULL || kk0 + u < W) {
                     const T term = gj_term(vi, vv[u], WC ? wc : wv[u]);
                     acc.add(((kk0 + u) * L + l < d) ? term : T(0));
                 }
             }
         }
-    }
+    };
+    int kk0 = 0;
+    if constexpr (sizeof(T) == 4)   // FP64: the second copy of the loop costs more registers than it saves
+        for (; kk0 + kUnroll <= W; kk0 += kUnroll) batch(std::true_type{}, kk0);
+    for (; kk0 < W; kk0 += kUnroll) batch(std::false_type{}, kk0);
 }
 
 template <typename T, bool KAHAN>
