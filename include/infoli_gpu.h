@@ -3,11 +3,12 @@
  *
  * One context holds one network (or a batch of independent networks laid out
  * block-diagonally) resident in HBM: structure-of-arrays cell state, the
- * receiver-major neighbour lists in sliced-ELL form, per-cell g_CaL and
- * applied-current schedule.  infoli_advance() runs the per-timestep update
- * (gap-junction currents, then the 3-compartment Hodgkin-Huxley update with
- * forward-Euler voltages) on the GPU for any number of steps and returns the
- * axon-voltage trace.
+ * receiver-major neighbour lists in the layout the chosen gather reads
+ * (sliced ELL, per-block staged tables or receiver-block x sender-chunk
+ * tiles), per-cell g_CaL and applied-current schedule.  infoli_advance()
+ * runs the per-timestep update (gap-junction currents, then the 3-compartment
+ * Hodgkin-Huxley update with forward-Euler voltages) on the GPU for any
+ * number of steps and returns the axon-voltage trace.
  *
  * The reference has no library boundary: its step is inline in main()'s loop
  * (src/openmp/infoli.c:480-698) over globals (src/openmp/infoli.h:43-72) and
