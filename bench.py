@@ -63,24 +63,46 @@ def algorithmic_bytes_per_step(n, nnz, prec):
     return per_entry * nnz + per_cell * n
 
 
-def cpu_baseline(wl, target_s=12.0):
+def _bind_openmp_threads():
+    """One OpenMP thread per core, packed (OMP_PLACES=cores,
+    OMP_PROC_BIND=close), unless the caller chose otherwise.  libgomp reads
+    these when it loads, so they only take effect before the first OpenMP
+    library is loaded into this process."""
+    try:
+        if "libgomp" in open("/proc/self/maps").read():
+            return False
+    except OSError:
+        return False
+    os.environ.setdefault("OMP_PLACES", "cores")
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    return True
+
+
+def cpu_baseline(wl, target_s=18.0):
+    """The oracle's OpenMP restatement of the reference step on this host's
+    cores: the FP32 rate is the median of three runs over steps [0, k) with k
+    sized to about target_s / 4 seconds each; one FP64 run of the same steps
+    is timed beside it.  Returns (cpu_baseline object, FP64 rate)."""
+    bound = _bind_openmp_threads()
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     n = wl["n"]
     threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     threads = max(1, min(threads, int(os.environ.get("OMP_NUM_THREADS", threads))))
 
-    def run(k):
+    def run(prec, k):
         st = O.initial_state(n)
         st[O.G_CAL] = wl["gcal"]
-        _, el = O.run(O.FP32, st, k, wl["row_ptr"], wl["col"], wl["w"],
+        _, el = O.run(prec, st, k, wl["row_ptr"], wl["col"], wl["w"],
                       iapp_base=wl["iapp_base"], mask=wl["mask"], amp=wl["amp"],
                       t0=wl["t0"], t1=wl["t1"], stride=100, nthreads=threads, trace=True)
         return el
 
-    probe = run(2)
-    k = int(max(2, min(2000, target_s / max(probe / 2, 1e-6))))
-    el = run(k)
+    probe = run(O.FP32, 2)
+    k = int(max(2, min(2000, target_s / 4 / max(probe / 2, 1e-6))))
+    runs = sorted(run(O.FP32, k) for _ in range(3))
+    el = runs[1]
+    el64 = run(O.FP64, k)
     cpu = platform.processor() or platform.machine()
     try:
         for line in open("/proc/cpuinfo"):
@@ -89,10 +111,13 @@ def cpu_baseline(wl, target_s=12.0):
                 break
     except OSError:
         pass
-    return {"value": n * k / el, "unit": "cell-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{wl['name']}: steps [0, {k}) of {wl['steps']}, all {n} cells, FP32 "
-                      f"OpenMP restatement of src/openmp/infoli.c:496-679 (oracle/infoli_oracle.c), "
-                      f"{threads} threads on {cpu}; {el:.2f} s"}
+    binding = "OMP_PLACES=cores OMP_PROC_BIND=close" if bound else "OpenMP binding as inherited"
+    return ({"value": n * k / el, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+             "sample": f"{wl['name']}: steps [0, {k}) of {wl['steps']}, all {n} cells, FP32 "
+                       f"OpenMP restatement of src/openmp/infoli.c:496-679 (oracle/infoli_oracle.c, "
+                       f"gcc -O3 -march=x86-64-v3), {threads} threads on {cpu}, {binding}; median of "
+                       f"3 runs: {', '.join(f'{r:.2f}' for r in runs)} s"},
+            n * k / el64)
 
 
 def per_network_fp64(I, wl, steps, warmup):
@@ -251,8 +276,9 @@ def main():
     cpu = None
     if not args.no_cpu:
         print("# CPU baseline", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(wl)
+        cpu, cpu64 = cpu_baseline(wl)
         extra["gpu_over_cpu"] = value / cpu["value"]
+        extra["cpu_fp64_cell_steps_per_s"] = cpu64
 
     out = {
         "metric": METRIC,

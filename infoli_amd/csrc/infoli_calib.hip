@@ -8,6 +8,11 @@
  *                                    `table` floats (global memory) and, when
  *                                    it fits, from LDS
  *   infoli_calib.x exp               throughput of the hardware exp2 (v_exp_f32)
+ *   infoli_calib.x launch            per-kernel cost inside a hipGraph, in the
+ *                                    sparse 100k step's launch shapes: an empty
+ *                                    kernel, a 7.2 MB producer, a consumer that
+ *                                    reads it and writes 12.8 MB, and the two
+ *                                    as a dependent pair
  *
  * Every index is drawn in range on the host; grids are sized to the buffers.
  */
@@ -95,6 +100,28 @@ __global__ void exp_rate(float *out, int iters)
         d = __builtin_amdgcn_exp2f(d) * -0.5f;
     }
     out[(long)blockIdx.x * blockDim.x + threadIdx.x] = a + b + c + d;
+}
+
+/* Launch-cost kernels.  The producer has the tile kernel's shape (441
+ * workgroups of 1024 threads, one 16-byte partial per thread); the consumer
+ * has the compartment kernel's (256-thread groups over 100k cells' 128 bytes
+ * of state). */
+__global__ __launch_bounds__(1024) void launch_empty(float4 *p)
+{
+    if (p == nullptr && threadIdx.x == 12345) p[0] = float4{};
+}
+
+__global__ __launch_bounds__(1024) void launch_produce(float4 *p)
+{
+    const size_t i = (size_t)blockIdx.x * 1024 + threadIdx.x;
+    p[i] = float4{1.f, 2.f, 3.f, (float)i};
+}
+
+__global__ __launch_bounds__(256) void launch_consume(const float4 *p, float4 *st, size_t np, size_t ns)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const float4 a = i < np ? p[i] : float4{};
+    if (i < ns) st[i] = a;
 }
 
 static float time_ms(hipEvent_t e0, hipEvent_t e1)
@@ -196,8 +223,48 @@ int main(int argc, char **argv)
             const float ms = time_ms(e0, e1);
             if (rep) printf("exp2 (+1 mul each): %.1f G exp/s\n", (double)nthreads * iters * 4 / ms / 1e6);
         }
+    } else if (!strcmp(what, "launch")) {
+        const int G = 441, K = 200;
+        const size_t np = (size_t)G * 1024, ns = 100000 * 8;
+        const dim3 cg((unsigned)((ns + 255) / 256));
+        float4 *part, *st;
+        CHK(hipMalloc(&part, np * 16));
+        CHK(hipMalloc(&st, ns * 16));
+        CHK(hipMemset(part, 0, np * 16));
+        hipStream_t s;
+        CHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        const char *names[] = {"empty 441x1024", "producer 7.2 MB", "consumer 7.2 MB in, 12.8 MB out",
+                               "producer -> consumer pair"};
+        for (int v = 0; v < 4; v++) {
+            hipGraph_t g;
+            hipGraphExec_t ge;
+            CHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            for (int k = 0; k < K; k++) {
+                if (v == 0) hipLaunchKernelGGL(launch_empty, dim3(G), dim3(1024), 0, s, part);
+                if (v == 1 || v == 3) hipLaunchKernelGGL(launch_produce, dim3(G), dim3(1024), 0, s, part);
+                if (v >= 2) hipLaunchKernelGGL(launch_consume, cg, dim3(256), 0, s, (const float4 *)part, st, np, ns);
+            }
+            CHK(hipStreamEndCapture(s, &g));
+            CHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+            float best = 1e30f;
+            for (int rep = 0; rep < 3; rep++) {
+                CHK(hipGraphLaunch(ge, s));
+                CHK(hipStreamSynchronize(s));
+                CHK(hipEventRecord(e0, s));
+                CHK(hipGraphLaunch(ge, s));
+                CHK(hipEventRecord(e1, s));
+                const float ms = time_ms(e0, e1);
+                best = ms < best ? ms : best;
+            }
+            printf("graph of %d x %-34s %.2f us each\n", K, names[v], best * 1000 / K);
+            CHK(hipGraphExecDestroy(ge));
+            CHK(hipGraphDestroy(g));
+        }
+        CHK(hipStreamDestroy(s));
+        CHK(hipFree(part));
+        CHK(hipFree(st));
     } else {
-        fprintf(stderr, "usage: infoli_calib.x copy [MiB] | gather [table] | exp\n");
+        fprintf(stderr, "usage: infoli_calib.x copy [MiB] | gather [table] | exp | launch\n");
         return 2;
     }
     return 0;
