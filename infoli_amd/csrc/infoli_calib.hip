@@ -4,9 +4,12 @@
  *   infoli_calib.x copy   [MiB]      plain device copy of a known size: the
  *                                    reference point for scaling rocprofv3's
  *                                    FETCH_SIZE / WRITE_SIZE counters
- *   infoli_calib.x gather [table]    random 4-byte gathers from a table of
+ *   infoli_calib.x gather [table] [xcd]
+ *                                    random 4-byte gathers from a table of
  *                                    `table` floats (global memory) and, when
- *                                    it fits, from LDS
+ *                                    it fits, from LDS; with `xcd`, each
+ *                                    workgroup reads one eighth of the table,
+ *                                    chosen by its XCD
  *   infoli_calib.x exp               throughput of the hardware exp2 (v_exp_f32)
  *   infoli_calib.x launch            per-kernel cost inside a hipGraph, in the
  *                                    sparse 100k step's launch shapes: an empty
@@ -181,8 +184,14 @@ int main(int argc, char **argv)
         const long ne = nthreads * per;
         std::vector<int> hidx(ne);
         std::mt19937 rng(1);
-        std::uniform_int_distribution<int> U(0, (int)nv - 1);
-        for (long i = 0; i < ne; i++) hidx[i] = U(rng);
+        // "xcd": workgroup b reads only slice b % 8 of the table.  Workgroups
+        // are dealt to the 8 XCDs round-robin, so each XCD's L2 sees one
+        // eighth of the table.
+        const bool xcd = argc > 3 && !strcmp(argv[3], "xcd") && nv >= 8;
+        const long slice = xcd ? nv / 8 : nv;
+        std::uniform_int_distribution<int> U(0, (int)slice - 1);
+        for (long i = 0; i < ne; i++)
+            hidx[i] = U(rng) + (xcd ? (int)(((i % nthreads) / 256) % 8 * slice) : 0);
         std::vector<float> hv(nv, 1.0f);
         int *idx;
         float *v, *out;
@@ -197,7 +206,9 @@ int main(int argc, char **argv)
             CHK(hipGetLastError());
             CHK(hipEventRecord(e1));
             const float ms = time_ms(e0, e1);
-            if (rep) printf("gather global, table %ld floats: %.3f ms, %.1f G gathers/s\n", nv, ms, ne / ms / 1e6);
+            if (rep)
+                printf("gather global, table %ld floats%s: %.3f ms, %.1f G gathers/s\n", nv,
+                       xcd ? " (one slice per XCD)" : "", ms, ne / ms / 1e6);
         }
         if (nv * 4 <= 64 * 1024) {
             for (int rep = 0; rep < 2; rep++) {
@@ -264,7 +275,7 @@ int main(int argc, char **argv)
         CHK(hipFree(part));
         CHK(hipFree(st));
     } else {
-        fprintf(stderr, "usage: infoli_calib.x copy [MiB] | gather [table] | exp | launch\n");
+        fprintf(stderr, "usage: infoli_calib.x copy [MiB] | gather [table] [xcd] | exp | launch\n");
         return 2;
     }
     return 0;

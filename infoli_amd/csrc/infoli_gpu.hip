@@ -245,6 +245,7 @@ StepArgs<T> make_args(const infoli_ctx *c, int64_t stride)
         a.part = c->d_part;
         a.ntiles = (int32_t)c->tl_ntiles;
         a.nsa = (int32_t)c->tl_nsa;
+        a.fast_math = c->exec.fast_math ? 1 : 0;
         a.fp64_deg = hyb_active(c) ? (int32_t)std::min<int64_t>(c->fp64_deg, INT32_MAX) : 0;
         a.rcells = c->tiled_R;
         a.ccells = (int32_t)c->tl_C;
@@ -321,9 +322,12 @@ const void *const k_tile_gj_hyb_tab[3][5] = {
     INFOLI_L_TABLE(k_tile_gj, float, true, 1024, true),
 };
 const void *const k_gap_hyb_tab[5] = INFOLI_L_TABLE(k_gap_currents, float, true, true);
-const void *const k_tile_dend_tab[3] = {
-    (const void *)k_tile_dend<double, false>, (const void *)k_tile_dend<float, false>,
-    (const void *)k_tile_dend<float, true>,
+/* [fast math][precision] */
+const void *const k_tile_dend_tab[2][3] = {
+    {(const void *)k_tile_dend<double, false, false>, (const void *)k_tile_dend<float, false, false>,
+     (const void *)k_tile_dend<float, true, false>},
+    {(const void *)k_tile_dend<double, false, false>, (const void *)k_tile_dend<float, false, true>,
+     (const void *)k_tile_dend<float, true, true>},
 };
 const void *const k_gap_tab[3][5] = {
     INFOLI_L_TABLE(k_gap_currents, double, false),
@@ -371,7 +375,7 @@ int launch_step(infoli_ctx *c, const void *args, int64_t k)
             // dendrites (which need the tiles' partials) after
             HIPCHK(hipLaunchKernel(fg, dim3((unsigned)(c->tl_ntiles + c->tl_nsa)), dim3(c->exec.block), kargs, lds,
                                    c->stream));
-            HIPCHK(hipLaunchKernel(k_tile_dend_tab[c->prec], dim3((unsigned)(c->npad / kBlock)), dim3(kBlock), kargs,
+            HIPCHK(hipLaunchKernel(k_tile_dend_tab[fm][c->prec], dim3((unsigned)(c->npad / kBlock)), dim3(kBlock), kargs,
                                    0, c->stream));
             return INFOLI_OK;
         }
@@ -838,7 +842,24 @@ void default_exec(infoli_ctx *c)
     const int64_t nrb = (c->n + R - 1) / R;
     const int64_t want = (L0 >= 4 ? 2 : 1) * (int64_t)c->num_cus;
     const int64_t nch_min = std::max<int64_t>(1, (want + nrb - 1) / nrb);
-    const int64_t C = std::min<int64_t>(C0, round_up((c->n + nch_min - 1) / nch_min, 64));
+    int64_t C = std::min<int64_t>(C0, round_up((c->n + nch_min - 1) / nch_min, 64));
+    bool even_tiles = false;
+    if (L0 < 4 && C == C0 && nrb <= c->num_cus) {
+        // sparse rows and more senders than 48 KiB chunks cover at that tile
+        // count: larger chunks, as many as give every CU the same number of
+        // tiles (one, or two when one tile's chunk would not fit the LDS);
+        // uneven rounds leave CUs that finish a tile early idle, and fewer
+        // chunks mean fewer partial sums
+        for (int64_t k = 1; k <= 2 && !even_tiles; k++) {
+            const int64_t nch = k * c->num_cus / nrb;
+            if (nch < 1) continue;
+            const int64_t Cb = round_up((c->n + nch - 1) / nch, 64);
+            if (Cb * (int64_t)c->tsize <= 156 * 1024 && k * Cb * (int64_t)c->tsize <= 160 * 1024) {
+                C = std::max<int64_t>(C, Cb);
+                even_tiles = true;
+            }
+        }
+    }
     const int kb = (int)std::max<int64_t>(1, (C * (int64_t)c->tsize + 1023) / 1024);
     const int64_t Ck = tiled_chunk(c, kb);
     const int L = lanes_for(in_tile_degree(R, Ck));
@@ -851,15 +872,19 @@ void default_exec(infoli_ctx *c)
         c->exec.lanes_per_cell = L;
         c->exec.tile_cells = R;
         // somata and axons beside the tiles when the tiles need more than
-        // one round of workgroup slots (they then fill the last round);
-        // with a single round they would only lengthen it
+        // one round of workgroup slots (they then fill the last round), or,
+        // with chunks sized for even rounds, only when every CU holds one
+        // tile and has a slot to spare; in a single uneven round, or in
+        // full even rounds, they would only lengthen it
         const void *fg = k_tile_gj_tab[c->prec][ntidx(1024)][lidx(L)];
         const size_t lds = (size_t)Ck * c->tsize;
         allow_dyn_lds(fg, lds);
         int per_cu = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fg, 1024, lds) != hipSuccess) per_cu = 1;
+        per_cu = std::max(per_cu, 1);
         const double ntiles = tpb * (double)((c->n + R - 1) / R);
-        c->exec.ode_split = ntiles > (double)std::max(per_cu, 1) * c->num_cus ? 2 : 1;
+        const bool spare = even_tiles && per_cu >= 2 && ntiles <= (double)c->num_cus;
+        c->exec.ode_split = (even_tiles ? spare : ntiles > (double)per_cu * c->num_cus) ? 2 : 1;
         return;
     }
     // staged gather when a 64-cell group fits and the lists stay in the
@@ -1362,8 +1387,6 @@ int check_exec(const infoli_ctx *c, infoli_exec &x)
         return infoli_fail(INFOLI_ERR_ARG, "fast_math is available with the tiled gather only");
     if (x.ode_split < 0 || x.ode_split > 2)
         return infoli_fail(INFOLI_ERR_ARG, "ode_split %d not in {0, 1, 2}", x.ode_split);
-    if (x.fast_math && x.ode_split == 2)
-        return infoli_fail(INFOLI_ERR_ARG, "fast_math is available with ode_split 0 or 1 only");
     if (hyb_active(c) && x.gather != GATHER_TILED)
         return infoli_fail(INFOLI_ERR_ARG, "cells with at least %lld neighbours sum FP64 gap-junction terms, "
                                            "which the tiled gather implements; set the FP64 degree to 0 "
@@ -1759,8 +1782,10 @@ extern "C" int infoli_autotune(infoli_t *c, int64_t trial_steps, infoli_exec *ch
         if (lidx(L) < 0) continue;
         for (int R : {R0 / 2, R0, R0 * 2}) {
             if (R < 64 || R > 16384 || R % 64) continue;
-            for (int kb : {std::max(1, kb0 / 2), kb0}) {
-                if (kb > 156) continue;
+            const int kbs[3] = {std::max(1, kb0 / 2), kb0, std::min(156, kb0 * 2)};
+            for (int i = 0; i < 3; i++) {
+                const int kb = kbs[i];
+                if (kb > 156 || (i > 0 && kb == kbs[i - 1])) continue;
                 for (int split : {1, 2}) {
                     const infoli_exec e{INFOLI_EXEC_GRAPH, L, 1024, spl, GATHER_TILED, kb, R, split, 0};
                     if (memcmp(&e, &h, sizeof e) != 0) cands.push_back(e);

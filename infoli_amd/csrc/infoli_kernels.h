@@ -102,6 +102,8 @@ struct StepArgs {
                                // neighbours sum FP64 terms (0 = none)
     int32_t ntiles;            // tiles (workgroups past them run somata and axons)
     int32_t nsa;               // soma/axon workgroups (each loops over cell groups)
+    int32_t fast_math;         // FP32 somata and axons of those workgroups with the
+                               // hardware exp2 and reciprocal
     int32_t rcells;            // receivers per block (multiple of 64)
     int32_t ccells;            // senders per chunk
     Model<T> m;
@@ -1124,12 +1126,12 @@ __global__ __launch_bounds__(192) void k_tile_ode3(StepArgs<T> a, int64_t k)
 
 /* Dendrites only (their somata and axons ran beside the tiles), one thread
  * per cell. */
-template <typename T, bool KAHAN>
+template <typename T, bool KAHAN, bool FM>
 __global__ __launch_bounds__(kBlock) void k_tile_dend(StepArgs<T> a, int64_t k)
 {
     const int64_t t = (a.d_step ? *a.d_step : 0) + k;
     const int64_t cell = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (cell < a.n) tile_dend_cell<T, KAHAN, false>(a, cell, t);
+    if (cell < a.n) tile_dend_cell<T, KAHAN, FM>(a, cell, t);
 }
 
 /* One tile: copy the sender chunk's V_dend to LDS, then each receiver's
@@ -1150,8 +1152,13 @@ __global__ __launch_bounds__(NT, (sizeof(T) == 4 && !HYB) ? 8 : 1) void k_tile_g
         for (int64_t g = (int64_t)blockIdx.x - a.ntiles; g * (NT / 2) < a.n; g += a.nsa) {
             const int64_t cell = (g * (NT / 128) + (wv >> 1)) * 64 + (threadIdx.x & 63);
             if (cell < a.n) {
-                if (wv & 1) tile_axon_cell<T, false>(a, cell, t);
-                else tile_soma_cell<T, false>(a, cell, t);
+                if (a.fast_math) {
+                    if (wv & 1) tile_axon_cell<T, true>(a, cell, t);
+                    else tile_soma_cell<T, true>(a, cell, t);
+                } else {
+                    if (wv & 1) tile_axon_cell<T, false>(a, cell, t);
+                    else tile_soma_cell<T, false>(a, cell, t);
+                }
             }
         }
         return;

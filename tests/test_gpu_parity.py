@@ -144,21 +144,22 @@ def test_exec_strategies_agree(mode, mname, L, gather, kb, gname, prec):
     assert np.abs(vd - st[O.V_DEND]).max() < tol
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32", "fp32k"])
+@pytest.mark.parametrize("prec,fm", [("fp64", 0), ("fp32", 0), ("fp32k", 0), ("fp32", 1), ("fp32k", 1)])
 @pytest.mark.parametrize("block", [256, 1024])
-def test_compartment_splits_bitwise_equal(prec, block):
+def test_compartment_splits_bitwise_equal(prec, fm, block):
     """The three placements of the compartment update (one thread per cell;
     one wavefront per compartment; somata and axons beside the tiles,
     dendrites after) run the same arithmetic on the same start-of-step
-    values, so they agree bit for bit, trace included."""
-    say(f"ode_split {prec} block {block}")
+    values, so they agree bit for bit, trace included, with either
+    compartment arithmetic."""
+    say(f"ode_split {prec} fast_math {fm} block {block}")
     n = 2000
     rp, col, w, gcal, iapp = _random_case(n, p=0.05, seed=17)
     outs = []
     for split in (0, 1, 2):
         net = make_net(n, prec, rp, col, w, gcal, iapp)
         net.set_exec(mode=I.EXEC_GRAPH, gather=I.GATHER_TILED, lds_kb=2, tile_cells=512, block=block,
-                     lanes_per_cell=2, ode_split=split, steps_per_launch=40)
+                     lanes_per_cell=2, ode_split=split, steps_per_launch=40, fast_math=fm)
         tr = net.advance(173, trace_stride=3)
         outs.append((tr, np.stack([net.get_state(v) for v in I.VARS])))
         net.close()
@@ -173,7 +174,11 @@ def test_compartment_split_options_checked():
     with pytest.raises(I.InfoliError):
         net.set_exec(gather=I.GATHER_TILED, ode_split=3)
     with pytest.raises(I.InfoliError):
-        net.set_exec(gather=I.GATHER_TILED, ode_split=2, fast_math=1)
+        net.set_exec(gather=I.GATHER_TILED, ode_split=-1)
+    with pytest.raises(I.InfoliError):
+        net.set_exec(gather=I.GATHER_DIRECT, fast_math=1)
+    net.set_exec(gather=I.GATHER_TILED, ode_split=2, fast_math=1)
+    assert net.exec["ode_split"] == 2 and net.exec["fast_math"] == 1
     net.close()
 
 
