@@ -821,48 +821,49 @@ void default_exec(infoli_ctx *c)
     c->exec.fast_math = 0;
     c->exec.gather = GATHER_DIRECT;
     if (mean_deg < 4 && !hyb_active(c)) return;
-    // tiled: lanes per receiver from the mean in-tile degree of a first
-    // layout guess (about 16 junctions per lane), receivers per tile
-    // inversely, sender chunks small enough for one tile per CU (two when
-    // the in-tile degree is high, so partial sums stay cheap), then the
-    // lanes again from the in-tile degree of that layout
+    // tiled: receivers per tile from the mean in-tile degree of a first
+    // layout guess (2048 / L0 with L0 about one lane per 16-32 junctions),
+    // sender chunks small enough for one tile per CU (two when the in-tile
+    // degree is high, so partial sums stay cheap) or sized for even rounds,
+    // then the lanes from the in-tile degree of that layout, about one lane
+    // per 32-64 junctions (longer runs per lane beat more lanes merging)
     const int64_t C0 = tiled_chunk(c, 48);
     const int R0 = 2048;
     const auto in_tile_degree = [c](int R, int64_t C) {
         const double tiles = tiles_per_block(c, R, C) * (double)((c->n + R - 1) / R);
         return tiles > 0 ? (double)c->col.size() / (tiles * R) : 0.0;
     };
-    const auto lanes_for = [](double dc) {
+    const auto lanes_for = [](double dc, double per_lane) {
         int L = 1;
-        while (L < 16 && 32.0 * L <= dc) L *= 2;
+        while (L < 16 && per_lane * L <= dc) L *= 2;
         return L;
     };
-    const int L0 = lanes_for(in_tile_degree(R0, C0));
+    const int L0 = lanes_for(in_tile_degree(R0, C0), 32.0);
     const int R = std::max(64, 2048 / L0);
     const int64_t nrb = (c->n + R - 1) / R;
     const int64_t want = (L0 >= 4 ? 2 : 1) * (int64_t)c->num_cus;
     const int64_t nch_min = std::max<int64_t>(1, (want + nrb - 1) / nrb);
     int64_t C = std::min<int64_t>(C0, round_up((c->n + nch_min - 1) / nch_min, 64));
     bool even_tiles = false;
-    if (L0 < 4 && C == C0 && nrb <= c->num_cus) {
-        // sparse rows and more senders than 48 KiB chunks cover at that tile
-        // count: larger chunks, as many as give every CU the same number of
-        // tiles (one, or two when one tile's chunk would not fit the LDS);
-        // uneven rounds leave CUs that finish a tile early idle, and fewer
-        // chunks mean fewer partial sums
+    if (nrb <= c->num_cus) {
+        // chunks as large as give every CU the same number of tiles (one,
+        // or two when one tile's chunk would not fit the LDS), when that is
+        // no smaller than the chunk above: uneven rounds leave the CUs that
+        // finish early idle, and fewer chunks mean fewer partial sums
         for (int64_t k = 1; k <= 2 && !even_tiles; k++) {
             const int64_t nch = k * c->num_cus / nrb;
             if (nch < 1) continue;
             const int64_t Cb = round_up((c->n + nch - 1) / nch, 64);
-            if (Cb * (int64_t)c->tsize <= 156 * 1024 && k * Cb * (int64_t)c->tsize <= 160 * 1024) {
-                C = std::max<int64_t>(C, Cb);
+            if (Cb >= C && Cb * (int64_t)c->tsize <= 156 * 1024 && k * Cb * (int64_t)c->tsize <= 160 * 1024) {
+                C = Cb;
                 even_tiles = true;
             }
         }
     }
     const int kb = (int)std::max<int64_t>(1, (C * (int64_t)c->tsize + 1023) / 1024);
     const int64_t Ck = tiled_chunk(c, kb);
-    const int L = lanes_for(in_tile_degree(R, Ck));
+    const double dc = in_tile_degree(R, Ck);
+    const int L = lanes_for(dc, 64.0);
     const double psz = c->prec == INFOLI_FP32 ? 4.0 : 8.0;
     const double tpb = tiles_per_block(c, R, Ck);
     if (hyb_active(c) || tpb * psz <= 0.5 * mean_deg * (2.0 + (double)c->tsize)) {
@@ -874,8 +875,10 @@ void default_exec(infoli_ctx *c)
         // somata and axons beside the tiles when the tiles need more than
         // one round of workgroup slots (they then fill the last round), or,
         // with chunks sized for even rounds, only when every CU holds one
-        // tile and has a slot to spare; in a single uneven round, or in
-        // full even rounds, they would only lengthen it
+        // tile and has a slot to spare and the tile rows are short (the
+        // tiles then wait on memory and leave issue slots free); in a single
+        // uneven round, in full even rounds or beside issue-bound tiles they
+        // would only lengthen the launch
         const void *fg = k_tile_gj_tab[c->prec][ntidx(1024)][lidx(L)];
         const size_t lds = (size_t)Ck * c->tsize;
         allow_dyn_lds(fg, lds);
@@ -883,7 +886,7 @@ void default_exec(infoli_ctx *c)
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fg, 1024, lds) != hipSuccess) per_cu = 1;
         per_cu = std::max(per_cu, 1);
         const double ntiles = tpb * (double)((c->n + R - 1) / R);
-        const bool spare = even_tiles && per_cu >= 2 && ntiles <= (double)c->num_cus;
+        const bool spare = even_tiles && per_cu >= 2 && ntiles <= (double)c->num_cus && dc < 64.0;
         c->exec.ode_split = (even_tiles ? spare : ntiles > (double)per_cu * c->num_cus) ? 2 : 1;
         return;
     }
