@@ -879,7 +879,8 @@ void default_exec(infoli_ctx *c)
         // tiles then wait on memory and leave issue slots free); in a single
         // uneven round, in full even rounds or beside issue-bound tiles they
         // would only lengthen the launch
-        const void *fg = k_tile_gj_tab[c->prec][ntidx(1024)][lidx(L)];
+        const void *fg = hyb_active(c) ? k_tile_gj_hyb_tab[ntidx(1024)][lidx(L)]
+                                       : k_tile_gj_tab[c->prec][ntidx(1024)][lidx(L)];
         const size_t lds = (size_t)Ck * c->tsize;
         allow_dyn_lds(fg, lds);
         int per_cu = 0;
