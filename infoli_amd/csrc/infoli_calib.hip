@@ -15,7 +15,11 @@
  *                                    sparse 100k step's launch shapes: an empty
  *                                    kernel, a 7.2 MB producer, a consumer that
  *                                    reads it and writes 12.8 MB, and the two
- *                                    as a dependent pair
+ *                                    as a dependent pair; then an 8 MiB
+ *                                    hand-over between two kernels whose
+ *                                    workgroup b reads what workgroup b + s
+ *                                    wrote, on the same XCD (s = 0, 8) or on
+ *                                    another (s = 1, 3, 4)
  *
  * Every index is drawn in range on the host; grids are sized to the buffers.
  */
@@ -125,6 +129,19 @@ __global__ __launch_bounds__(256) void launch_consume(const float4 *p, float4 *s
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const float4 a = i < np ? p[i] : float4{};
     if (i < ns) st[i] = a;
+}
+
+/* hand-over: workgroup b writes, then reads, chunk b of 1024 floats */
+__global__ __launch_bounds__(256) void handover_write(float *p)
+{
+    const size_t base = (size_t)blockIdx.x * 1024;
+    for (int i = threadIdx.x; i < 1024; i += 256) p[base + i] = (float)(base + i);
+}
+
+__global__ __launch_bounds__(256) void handover_read(const float *p, float *q, int shift, int nb)
+{
+    const size_t src = (size_t)((blockIdx.x + shift) % nb) * 1024, dst = (size_t)blockIdx.x * 1024;
+    for (int i = threadIdx.x; i < 1024; i += 256) q[dst + i] = p[src + i] + 1.0f;
 }
 
 static float time_ms(hipEvent_t e0, hipEvent_t e1)
@@ -271,6 +288,41 @@ int main(int argc, char **argv)
             CHK(hipGraphExecDestroy(ge));
             CHK(hipGraphDestroy(g));
         }
+        // hand-over of 8 MiB: 2048 workgroups, each reading the chunk that
+        // workgroup b + shift wrote (workgroups are dealt to the 8 XCDs
+        // round-robin, so shifts that are multiples of 8 stay on one XCD)
+        const int nb = 2048;
+        float *hp, *hq;
+        CHK(hipMalloc(&hp, (size_t)nb * 4096));
+        CHK(hipMalloc(&hq, (size_t)nb * 4096));
+        CHK(hipMemset(hp, 0, (size_t)nb * 4096));
+        for (int shift : {0, 8, 1, 3, 4}) {
+            hipGraph_t g;
+            hipGraphExec_t ge;
+            CHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            for (int k = 0; k < K; k++) {
+                hipLaunchKernelGGL(handover_write, dim3(nb), dim3(256), 0, s, hp);
+                hipLaunchKernelGGL(handover_read, dim3(nb), dim3(256), 0, s, (const float *)hp, hq, shift, nb);
+            }
+            CHK(hipStreamEndCapture(s, &g));
+            CHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+            float best = 1e30f;
+            for (int rep = 0; rep < 3; rep++) {
+                CHK(hipGraphLaunch(ge, s));
+                CHK(hipStreamSynchronize(s));
+                CHK(hipEventRecord(e0, s));
+                CHK(hipGraphLaunch(ge, s));
+                CHK(hipEventRecord(e1, s));
+                const float ms = time_ms(e0, e1);
+                best = ms < best ? ms : best;
+            }
+            printf("8 MiB hand-over, reader b reads writer b + %d (%s XCD): %.2f us per pair\n", shift,
+                   shift % 8 ? "another" : "the same", best * 1000 / K);
+            CHK(hipGraphExecDestroy(ge));
+            CHK(hipGraphDestroy(g));
+        }
+        CHK(hipFree(hp));
+        CHK(hipFree(hq));
         CHK(hipStreamDestroy(s));
         CHK(hipFree(part));
         CHK(hipFree(st));
