@@ -839,12 +839,13 @@ void default_exec(infoli_ctx *c)
         return L;
     };
     const int L0 = lanes_for(in_tile_degree(R0, C0), 32.0);
-    const int R = std::max(64, 2048 / L0);
-    const int64_t nrb = (c->n + R - 1) / R;
+    int R = std::max(64, 2048 / L0);
+    int64_t nrb = (c->n + R - 1) / R;
     const int64_t want = (L0 >= 4 ? 2 : 1) * (int64_t)c->num_cus;
     const int64_t nch_min = std::max<int64_t>(1, (want + nrb - 1) / nrb);
     int64_t C = std::min<int64_t>(C0, round_up((c->n + nch_min - 1) / nch_min, 64));
     bool even_tiles = false;
+    int64_t even_k = 0;   // tiles per CU of the even rounds
     if (nrb <= c->num_cus) {
         // chunks as large as give every CU the same number of tiles (one,
         // or two when one tile's chunk would not fit the LDS), when that is
@@ -857,7 +858,27 @@ void default_exec(infoli_ctx *c)
             if (Cb >= C && Cb * (int64_t)c->tsize <= 156 * 1024 && k * Cb * (int64_t)c->tsize <= 160 * 1024) {
                 C = Cb;
                 even_tiles = true;
+                even_k = k;
             }
+        }
+    }
+    if (even_k == 2 && 2 * R <= 16384) {
+        // two tiles per CU by the LDS may still be one by the registers (the
+        // FP64 tile kernel holds one 1024-thread workgroup per CU): then
+        // twice the receivers per block give one even round instead of two
+        const void *fk = hyb_active(c) ? k_tile_gj_hyb_tab[ntidx(1024)][lidx(L0)]
+                                       : k_tile_gj_tab[c->prec][ntidx(1024)][lidx(L0)];
+        const size_t lds = (size_t)C * c->tsize;
+        allow_dyn_lds(fk, lds);
+        int pc = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, fk, 1024, lds) != hipSuccess) pc = 1;
+        const int64_t nrb2 = (c->n + 2 * R - 1) / (2 * R);
+        const int64_t nch2 = c->num_cus / nrb2;
+        const int64_t C2 = nch2 >= 1 ? round_up((c->n + nch2 - 1) / nch2, 64) : 0;
+        if (pc < 2 && nch2 >= 1 && C2 * (int64_t)c->tsize <= 156 * 1024) {
+            R *= 2;
+            nrb = nrb2;
+            C = C2;
         }
     }
     const int kb = (int)std::max<int64_t>(1, (C * (int64_t)c->tsize + 1023) / 1024);
