@@ -89,6 +89,7 @@ struct infoli_ctx {
     int tiled_L = 0, tiled_kb = 0, tiled_R = 0;
     int64_t tl_ntiles = 0, tl_nrb = 0, tl_C = 0;
     int64_t tl_nsa = 1;     // soma/axon workgroups launched beside the tiles (ode_split 2)
+    bool tl_allwc = false;  // every tile has a constant conductance
     TileDesc *d_tiles = nullptr;
     int32_t *d_rb_tile0 = nullptr;
     int64_t *d_tsp = nullptr;
@@ -316,6 +317,11 @@ const void *const k_tile_ode3_tab[2][3] = {
      (const void *)k_tile_ode3<float, true, true>},
 };
 /* FP32 compensated with FP64 sums for high-degree cells: [threads][L] */
+/* every tile of constant conductance, FP32, 1024 threads: [compensated][L] */
+const void *const k_tile_gj_awc_tab[2][5] = {
+    INFOLI_L_TABLE(k_tile_gj, float, false, 1024, false, true),
+    INFOLI_L_TABLE(k_tile_gj, float, true, 1024, false, true),
+};
 const void *const k_tile_gj_hyb_tab[3][5] = {
     INFOLI_L_TABLE(k_tile_gj, float, true, 256, true),
     INFOLI_L_TABLE(k_tile_gj, float, true, 512, true),
@@ -347,6 +353,20 @@ size_t dyn_lds_bytes(const infoli_ctx *c)
     }
 }
 
+/* the tile kernel of the context's strategy and layout: the one whose rows
+ * prefetch their next ids when every tile has a constant conductance and
+ * the tiles run alone (measured: dense_10k 13.5 -> 12.4 us per step with
+ * the compartments after the tiles; batched_512 83.4 -> 90.8 us with the
+ * somata and axons beside them, so not then) */
+const void *tile_kernel(const infoli_ctx *c)
+{
+    const int li = lidx(c->exec.lanes_per_cell);
+    if (hyb_active(c)) return k_tile_gj_hyb_tab[ntidx(c->exec.block)][li];
+    if (c->tl_allwc && c->exec.ode_split != 2 && is_f32(c) && c->exec.block == 1024)
+        return k_tile_gj_awc_tab[c->prec == INFOLI_FP32_KAHAN ? 1 : 0][li];
+    return k_tile_gj_tab[c->prec][ntidx(c->exec.block)][li];
+}
+
 /* raise the dynamic-LDS limit of a kernel above 64 KiB when needed */
 void allow_dyn_lds(const void *fn, size_t bytes)
 {
@@ -362,8 +382,7 @@ int64_t n_blocks(const infoli_ctx *c)
 int launch_step(infoli_ctx *c, const void *args, int64_t k)
 {
     if (c->exec.gather == GATHER_TILED) {
-        const void *fg = hyb_active(c) ? k_tile_gj_hyb_tab[ntidx(c->exec.block)][lidx(c->exec.lanes_per_cell)]
-                                       : k_tile_gj_tab[c->prec][ntidx(c->exec.block)][lidx(c->exec.lanes_per_cell)];
+        const void *fg = tile_kernel(c);
         const int fm = c->exec.fast_math ? 1 : 0;
         const void *fo = k_tile_ode_tab[fm][c->prec];
         int64_t kk = k;
@@ -504,6 +523,7 @@ void free_tiled(infoli_ctx *c)
     c->d_tw = c->d_part = nullptr;
     c->tiled_valid = false;
     c->tl_ntiles = c->tl_nrb = c->tl_C = 0;
+    c->tl_allwc = false;
 }
 
 /* chunk of senders per tile for an LDS budget: a multiple of 64 cells */
@@ -624,6 +644,7 @@ int upload_tiled(infoli_ctx *c, int L, int R, int kb)
         HIPCHK(hipMemcpy(c->d_tw, tw.data(), sizeof(T) * tw.size(), hipMemcpyHostToDevice));
     }
     c->tl_ntiles = (int64_t)tiles.size();
+    c->tl_allwc = std::all_of(tiles.begin(), tiles.end(), [](const TileDesc &t) { return t.wconst != 0; });
     c->tl_nrb = nrb;
     c->tl_C = C;
     c->tiled_L = L;
@@ -718,8 +739,7 @@ int64_t soma_axon_workgroups(const infoli_ctx *c)
 {
     const int64_t per = c->exec.block / 2;
     const int64_t groups = std::max<int64_t>(1, (c->n + per - 1) / per);
-    const void *fg = hyb_active(c) ? k_tile_gj_hyb_tab[ntidx(c->exec.block)][lidx(c->exec.lanes_per_cell)]
-                                   : k_tile_gj_tab[c->prec][ntidx(c->exec.block)][lidx(c->exec.lanes_per_cell)];
+    const void *fg = tile_kernel(c);
     const size_t lds = (size_t)c->tl_C * c->tsize;
     allow_dyn_lds(fg, lds);
     int per_cu = 0;

@@ -455,8 +455,18 @@ __device__ __forceinline__ float gj_term(float vi, float vj, float w)
     const float K = -0.014426950408889634f; /* -log2(e) / 100 */
     float V = vi - vj;
     float e = __builtin_amdgcn_exp2f((V * V) * K);
-    float f = 0.8f * e + 0.2f;
+    float f = __builtin_fmaf(0.8f, e, 0.2f);
     return w * f * V;
+}
+
+/* the same for a conductance w known in advance: (0.8 w) e + 0.2 w as one
+ * fused multiply-add, with wa = 0.8 w and wb = 0.2 w */
+__device__ __forceinline__ float gj_term_c(float vi, float vj, float wa, float wb)
+{
+    const float K = -0.014426950408889634f; /* -log2(e) / 100 */
+    float V = vi - vj;
+    float e = __builtin_amdgcn_exp2f((V * V) * K);
+    return __builtin_fmaf(wa, e, wb) * V;
 }
 
 __device__ __forceinline__ double gj_term(double vi, double vj, double w)
@@ -936,51 +946,100 @@ __device__ __forceinline__ X ld_off(const X *__restrict__ p, uint32_t byte_off)
  * conductance wc. */
 template <typename T, bool KAHAN, int L, bool WC, bool HYB>
 __device__ __forceinline__ void tile_row_terms(Acc<T, KAHAN> &acc, double &accd, bool wide, T vi, int d, int l,
-                                               int W, int loff, const uint16_t *__restrict__ irow,
-                                               const T *__restrict__ wrow, T wc, const T *__restrict__ vch)
+                                               int W, int nvm, int kstart, int loff,
+                                               const uint16_t *__restrict__ irow, const T *__restrict__ wrow, T wc,
+                                               const T *__restrict__ vch)
 {
     const uint32_t oi = (uint32_t)loff * (uint32_t)sizeof(uint16_t);
     const uint32_t ow = (uint32_t)loff * (uint32_t)sizeof(T);
-    // one batch of kUnroll slots from kk0; FULL: all of them inside the
-    // slice width, so no slot needs a width check
-    const auto batch = [&](auto full, int kk0) {
-        constexpr bool FULL = decltype(full)::value;
+    const T wa = T(0.8) * wc, wb = T(0.2) * wc;
+    // one batch of kUnroll slots from kk0.  CHW: check the slice width W
+    // (slots past it do not exist); CHD: check the row's degree (slots past
+    // it are padding, read but not added)
+    const auto batch = [&](auto chw, auto chd, int kk0) {
+        constexpr bool CHW = decltype(chw)::value, CHD = decltype(chd)::value;
         const uint16_t *ib = irow + (int64_t)kk0 * 64;
-        const T *wb = wrow + (int64_t)kk0 * 64;
+        const T *wb_ = wrow + (int64_t)kk0 * 64;
         int jv[kUnroll];
         T wv[kUnroll], vv[kUnroll];
 #pragma unroll
         for (int u = 0; u < kUnroll; u++) {
-            if (FULL || kk0 + u < W) {
+            if (!CHW || kk0 + u < W) {
                 jv[u] = (int)ld_off(ib + u * 64, oi);
-                if constexpr (!WC) wv[u] = ld_off(wb + u * 64, ow);
+                if constexpr (!WC) wv[u] = ld_off(wb_ + u * 64, ow);
             }
         }
 #pragma unroll
         for (int u = 0; u < kUnroll; u++)
-            if (FULL || kk0 + u < W) vv[u] = vch[jv[u]];
+            if (!CHW || kk0 + u < W) vv[u] = vch[jv[u]];
         if (HYB && wide) {
 #pragma unroll
             for (int u = 0; u < kUnroll; u++) {
-                if (FULL || kk0 + u < W) {
+                if (!CHW || kk0 + u < W) {
                     const double term = gj_term((double)vi, (double)vv[u], (double)(WC ? wc : wv[u]));
-                    accd += ((kk0 + u) * L + l < d) ? term : 0.0;
+                    accd += (!CHD || (kk0 + u) * L + l < d) ? term : 0.0;
                 }
             }
         } else {
 #pragma unroll
             for (int u = 0; u < kUnroll; u++) {
-                if (FULL || kk0 + u < W) {
-                    const T term = gj_term(vi, vv[u], WC ? wc : wv[u]);
-                    acc.add(((kk0 + u) * L + l < d) ? term : T(0));
+                if (!CHW || kk0 + u < W) {
+                    T term;
+                    if constexpr (WC && sizeof(T) == 4) term = gj_term_c(vi, vv[u], wa, wb);
+                    else term = gj_term(vi, vv[u], WC ? wc : wv[u]);
+                    acc.add((!CHD || (kk0 + u) * L + l < d) ? term : T(0));
                 }
             }
         }
     };
+    int kk0 = kstart;
+    if constexpr (sizeof(T) == 4) {   // FP64: the extra copies of the loop cost more registers than they save
+        // slots below nvm (the fewest valid slots of any lane of the wave)
+        // need no check; then whole batches inside the width
+        const int nv = nvm < W ? nvm : W;
+        for (; kk0 + kUnroll <= nv; kk0 += kUnroll) batch(std::false_type{}, std::false_type{}, kk0);
+        for (; kk0 + kUnroll <= W; kk0 += kUnroll) batch(std::false_type{}, std::true_type{}, kk0);
+    }
+    for (; kk0 < W; kk0 += kUnroll) batch(std::true_type{}, std::true_type{}, kk0);
+}
+
+/* FP32 tile rows of constant conductance wc, the ids of the next batch
+ * loaded while the current one is summed (the unchecked slots below nvm),
+ * then the checked tail as above. */
+template <bool KAHAN, int L>
+__device__ __forceinline__ void tile_row_terms_pf(Acc<float, KAHAN> &acc, float vi, int d, int l, int W, int nvm,
+                                                  int loff, const uint16_t *__restrict__ irow, float wc,
+                                                  const float *__restrict__ vch)
+{
+    const uint32_t oi = (uint32_t)loff * (uint32_t)sizeof(uint16_t);
+    const float wa = 0.8f * wc, wb = 0.2f * wc;
+    const int nv = nvm < W ? nvm : W;
     int kk0 = 0;
-    if constexpr (sizeof(T) == 4)   // FP64: the second copy of the loop costs more registers than it saves
-        for (; kk0 + kUnroll <= W; kk0 += kUnroll) batch(std::true_type{}, kk0);
-    for (; kk0 < W; kk0 += kUnroll) batch(std::false_type{}, kk0);
+    if (kUnroll <= nv) {
+        int jv[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) jv[u] = (int)ld_off(irow + u * 64, oi);
+        for (; kk0 + kUnroll <= nv; kk0 += kUnroll) {
+            const bool more = kk0 + 2 * kUnroll <= nv;
+            int jn[kUnroll];
+            if (more) {
+#pragma unroll
+                for (int u = 0; u < kUnroll; u++) jn[u] = (int)ld_off(irow + (int64_t)(kk0 + kUnroll + u) * 64, oi);
+            }
+            float vv[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) vv[u] = vch[jv[u]];
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) acc.add(gj_term_c(vi, vv[u], wa, wb));
+            if (more) {
+#pragma unroll
+                for (int u = 0; u < kUnroll; u++) jv[u] = jn[u];
+            }
+        }
+    }
+    double unused = 0.0;
+    tile_row_terms<float, KAHAN, L, true, false>(acc, unused, false, vi, d, l, W, kk0, kk0, loff, irow, vch, wc,
+                                                 vch);   // (no conductances are read: wc)
 }
 
 template <typename T, bool KAHAN>
@@ -1136,9 +1195,11 @@ __global__ __launch_bounds__(kBlock) void k_tile_dend(StepArgs<T> a, int64_t k)
 
 /* One tile: copy the sender chunk's V_dend to LDS, then each receiver's
  * junctions from that chunk (L lanes per receiver, list order) into a
- * partial (sum, compensation) pair. */
-template <typename T, int L, bool KAHAN, int NT, bool HYB = false>
-__global__ __launch_bounds__(NT, (sizeof(T) == 4 && !HYB) ? 8 : 1) void k_tile_gj(StepArgs<T> a, int64_t k)
+ * partial (sum, compensation) pair.  AWC: every tile has a constant
+ * conductance (FP32 only), and the rows prefetch their next ids, which
+ * takes more registers (four waves per SIMD rather than eight). */
+template <typename T, int L, bool KAHAN, int NT, bool HYB = false, bool AWC = false>
+__global__ __launch_bounds__(NT, (sizeof(T) == 4 && !HYB) ? (AWC ? 4 : 8) : 1) void k_tile_gj(StepArgs<T> a, int64_t k)
 {
     extern __shared__ __align__(16) unsigned char dyn_lds[];
     T *vch = reinterpret_cast<T *>(dyn_lds);
@@ -1198,6 +1259,15 @@ __global__ __launch_bounds__(NT, (sizeof(T) == 4 && !HYB) ? 8 : 1) void k_tile_g
         }
         if (sl < td.nsl) {
             Acc<T, KAHAN> acc;
+            // the fewest valid slots of any lane in the wave (all lanes are
+            // active here): below it no slot needs the degree check
+            int nvm = 0;
+            if constexpr (sizeof(T) == 4) {
+                int m = d > l ? (d - l + L - 1) / L : 0;
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) m = min(m, __shfl_xor(m, off));
+                nvm = __builtin_amdgcn_readfirstlane(m);
+            }
             if (d > 0) {
                 const T vi = vin[cellb + rloc];
                 const int64_t base0 = a.tsp[td.sl0 + sl];
@@ -1206,12 +1276,14 @@ __global__ __launch_bounds__(NT, (sizeof(T) == 4 && !HYB) ? 8 : 1) void k_tile_g
                 // (sum, compensation) pair of floats
                 const bool wide = HYB && a.deg[cellb + rloc] >= a.fp64_deg;
                 double accd = 0.0;
-                if (td.wconst)
-                    tile_row_terms<T, KAHAN, L, true, HYB>(acc, accd, wide, vi, d, l, W, loff, a.tcol + base0,
-                                                           a.tw + base0, (T)td.wc, vch);
+                if constexpr (AWC && sizeof(T) == 4 && !HYB)
+                    tile_row_terms_pf<KAHAN, L>(acc, vi, d, l, W, nvm, loff, a.tcol + base0, (float)td.wc, vch);
+                else if (td.wconst)
+                    tile_row_terms<T, KAHAN, L, true, HYB>(acc, accd, wide, vi, d, l, W, nvm, 0, loff,
+                                                           a.tcol + base0, a.tw + base0, (T)td.wc, vch);
                 else
-                    tile_row_terms<T, KAHAN, L, false, HYB>(acc, accd, wide, vi, d, l, W, loff, a.tcol + base0,
-                                                            a.tw + base0, (T)td.wc, vch);
+                    tile_row_terms<T, KAHAN, L, false, HYB>(acc, accd, wide, vi, d, l, W, nvm, 0, loff,
+                                                            a.tcol + base0, a.tw + base0, (T)td.wc, vch);
                 if (HYB && wide) {
                     acc.s = (T)accd;
                     acc.c = (T)(accd - (double)acc.s);

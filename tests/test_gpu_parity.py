@@ -168,6 +168,35 @@ def test_compartment_splits_bitwise_equal(prec, fm, block):
         assert np.array_equal(st, outs[0][1])
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp32k"])
+@pytest.mark.parametrize("L", [1, 4, 8])
+@pytest.mark.parametrize("p", [1.0, 0.12])
+def test_constant_conductance_tiles(prec, L, p):
+    """Every junction of conductance 0.04 (the dense and batched networks):
+    1024-thread tiles with the compartments after them take the kernel
+    whose rows prefetch their next ids, 512-thread tiles the general one.
+    Same arithmetic in the same order, so the two agree bit for bit, and
+    both stay within tolerance of the oracle; the complete graph has rows
+    of one degree, 10% density rows of many."""
+    say(f"constant-conductance tiles {prec} L={L} p={p}")
+    n, steps = 1200, 200
+    rp, col, w = I.gen_symmetric(31, n, p, 0.04, 0.04)
+    assert np.all(w == 0.04)
+    gcal = I.gen_uniform(31, 2, n, 0.5, 1.1)
+    iapp = I.gen_uniform(31, 3, n, -1.0, 3.0)
+    outs = []
+    for block in (1024, 512):
+        net = make_net(n, prec, rp, col, w, gcal, iapp)
+        net.set_exec(mode=I.EXEC_GRAPH, gather=I.GATHER_TILED, lds_kb=2, tile_cells=256, block=block,
+                     lanes_per_cell=L, ode_split=1, steps_per_launch=50)
+        outs.append((net.advance(steps, trace_stride=3), np.stack([net.get_state(v) for v in I.VARS])))
+        net.close()
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
+    ref, _ = oracle_ref(prec, n, steps, rp, col, w, gcal, iapp, stride=3)
+    assert np.abs(outs[0][0] - ref).max() < 2e-4
+
+
 def test_compartment_split_options_checked():
     say("ode_split option checks")
     net = make_net(100, "fp32k")
