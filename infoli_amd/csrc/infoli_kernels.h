@@ -55,6 +55,9 @@ struct TileDesc {
     int64_t sl0;               // first slice in tsp
     int64_t rl0;               // first receiver slot in rlist / rdeg
     double wc;
+    int64_t s0;                // senders [s0, s0 + cn) staged in LDS; ids are relative to s0
+    int32_t cn;
+    int32_t pad;
 };
 
 template <typename T>
@@ -105,7 +108,6 @@ struct StepArgs {
     int32_t fast_math;         // FP32 somata and axons of those workgroups with the
                                // hardware exp2 and reciprocal
     int32_t rcells;            // receivers per block (multiple of 64)
-    int32_t ccells;            // senders per chunk
     Model<T> m;
 };
 
@@ -1231,8 +1233,8 @@ __global__ __launch_bounds__(NT, (sizeof(T) == 4 && !HYB) ? (AWC ? 4 : 8) : 1) v
     // scalar base with a 32-bit lane offset
     const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int cw = lane / L, l = lane % L;
-    const int64_t s0 = (int64_t)td.ch * a.ccells;
-    const int64_t cn = (a.n - s0 < a.ccells) ? a.n - s0 : a.ccells;
+    const int64_t s0 = td.s0;
+    const int64_t cn = td.cn;
     const int64_t cellb = (int64_t)td.rb * a.rcells;
     constexpr int NW = NT / 64;
     const int loff = cw * L + l;
