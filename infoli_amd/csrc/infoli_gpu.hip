@@ -95,6 +95,7 @@ struct infoli_ctx {
     int64_t *d_tsp = nullptr;
     uint16_t *d_rlist = nullptr, *d_rdeg = nullptr, *d_tcol = nullptr;
     void *d_tw = nullptr, *d_part = nullptr;
+    void *d_dpre = nullptr;   // ode_split 2: [2][npad] dendrite current sum before I_c, and I_h
 
     void *d_iapp_base = nullptr;
     uint8_t *d_mask = nullptr;
@@ -244,6 +245,7 @@ StepArgs<T> make_args(const infoli_ctx *c, int64_t stride)
         a.tcol = c->d_tcol;
         a.tw = (const T *)c->d_tw;
         a.part = c->d_part;
+        a.dpre = (T *)c->d_dpre;
         a.ntiles = (int32_t)c->tl_ntiles;
         a.nsa = (int32_t)c->tl_nsa;
         a.fast_math = c->exec.fast_math ? 1 : 0;
@@ -512,9 +514,11 @@ void free_staged(infoli_ctx *c)
  * list; and per cell its LDS row start. */
 void free_tiled(infoli_ctx *c)
 {
-    void *bufs[] = {c->d_tiles, c->d_rb_tile0, c->d_tsp, c->d_rlist, c->d_rdeg, c->d_tcol, c->d_tw, c->d_part};
+    void *bufs[] = {c->d_tiles, c->d_rb_tile0, c->d_tsp, c->d_rlist, c->d_rdeg, c->d_tcol, c->d_tw, c->d_part,
+                    c->d_dpre};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
+    c->d_dpre = nullptr;
     c->d_tiles = nullptr;
     c->d_rb_tile0 = nullptr;
     c->d_tsp = nullptr;
@@ -647,6 +651,7 @@ int upload_tiled(infoli_ctx *c, int L, int R, int kb)
     HIPCHK(hipMalloc(&c->d_tcol, sizeof(uint16_t) * std::max<size_t>(tcol.size(), 1)));
     HIPCHK(hipMalloc(&c->d_tw, sizeof(T) * std::max<size_t>(tw.size(), 1)));
     HIPCHK(hipMalloc(&c->d_part, psz * tiles.size() * (size_t)R));
+    HIPCHK(hipMalloc(&c->d_dpre, 2 * (size_t)c->npad * c->tsize));
     HIPCHK(hipMemcpy(c->d_tiles, tiles.data(), sizeof(TileDesc) * tiles.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_rb_tile0, rbt0.data(), sizeof(int32_t) * rbt0.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_tsp, tsp.data(), sizeof(int64_t) * tsp.size(), hipMemcpyHostToDevice));

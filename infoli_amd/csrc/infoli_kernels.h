@@ -107,6 +107,8 @@ struct StepArgs {
     int32_t nsa;               // soma/axon workgroups (each loops over cell groups)
     int32_t fast_math;         // FP32 somata and axons of those workgroups with the
                                // hardware exp2 and reciprocal
+    T *dpre;                   // ode_split 2: [2][npad] per cell the dendrite's current
+                               // sum before I_c, and I_h (written beside the tiles)
     int32_t rcells;            // receivers per block (multiple of 64)
     Model<T> m;
 };
@@ -161,10 +163,11 @@ __device__ __forceinline__ float D(float a, float b)
     else return a / b;
 }
 
-/* dendrite, OMP:561-593; updates q, r, s, ca, icah */
+/* dendrite, OMP:561-593, up to its voltage: updates q, r, s, ca, icah and
+ * returns the sum of the currents that precede I_c (I_h apart) */
 template <bool FM = false>
-__device__ __forceinline__ float ode_dend(const Model<float> &P, CellState32 &c, float Vd, float Vs,
-                                          float I_c, float iApp)
+__device__ __forceinline__ float ode_dend_pre(const Model<float> &P, CellState32 &c, float Vd, float Vs,
+                                              float &I_h_out)
 {
     // Dend H current OMP:561-564
     float q_inf = D<FM>(1, 1 + E<FM>((Vd + 80) / 4));
@@ -194,9 +197,29 @@ __device__ __forceinline__ float ode_dend(const Model<float> &P, CellState32 &c,
     float I_K_Ca = P.G_K_CA * c.s * (Vd - P.V_K);
     float I_ld = P.G_LD * (Vd - P.V_L);
     float I_h = P.G_H * c.q * (Vd - P.V_H);
-    float dVd_dt = D<FM>(-(I_CaH + I_sd + I_ld + I_K_Ca + I_c + I_h) + iApp, P.C_M);
     c.icah = I_CaH;
+    I_h_out = I_h;
+    // the reference sums left to right: ((((I_CaH + I_sd) + I_ld) + I_K_Ca) + I_c) + I_h,
+    // so the first four terms form one partial sum and I_c joins later
+    return I_CaH + I_sd + I_ld + I_K_Ca;
+}
+
+/* the dendrite voltage from the partial current sum of ode_dend_pre */
+template <bool FM = false>
+__device__ __forceinline__ float ode_dend_post(const Model<float> &P, float Vd, float pre, float I_h, float I_c,
+                                               float iApp)
+{
+    float dVd_dt = D<FM>(-(pre + I_c + I_h) + iApp, P.C_M);
     return P.DELTA * dVd_dt + Vd;
+}
+
+template <bool FM = false>
+__device__ __forceinline__ float ode_dend(const Model<float> &P, CellState32 &c, float Vd, float Vs,
+                                          float I_c, float iApp)
+{
+    float I_h;
+    const float pre = ode_dend_pre<FM>(P, c, Vd, Vs, I_h);
+    return ode_dend_post<FM>(P, Vd, pre, I_h, I_c, iApp);
 }
 
 /* soma, OMP:599-642; updates k, l, m, h, n, p, xs */
@@ -279,10 +302,10 @@ __device__ __forceinline__ float ode_axon(const Model<float> &P, CellState32 &c,
 /* compartment update, FP64: src/single_threaded/infoli.c:371-858          */
 /* ---------------------------------------------------------------------- */
 
-/* CompDend ST:371-428 (DendHCurr, DendCaCurr, DendKCurr, DendCal,
+/* CompDend ST:371-428, split as the FP32 form above (DendHCurr, DendCaCurr, DendKCurr, DendCal,
  * DendCurrVolt) */
-__device__ __forceinline__ double ode_dend(const Model<double> &P, CellState64 &c, double prevV_dend,
-                                           double prevV_soma, double I_c, double I_app)
+__device__ __forceinline__ double ode_dend_pre(const Model<double> &P, CellState64 &c, double prevV_dend,
+                                               double prevV_soma, double &I_h_out)
 {
     // DendHCurr ST:430-447
     double q_inf = 1 / (1 + exp((prevV_dend + 80) / 4));
@@ -312,9 +335,25 @@ __device__ __forceinline__ double ode_dend(const Model<double> &P, CellState64 &
     double I_K_Ca = P.G_K_CA * sK * (prevV_dend - P.V_K);
     double I_ld = P.G_LD * (prevV_dend - P.V_L);
     double I_h = P.G_H * q * (prevV_dend - P.V_H);
-    double dVd_dt = (-(I_CaH + I_sd + I_ld + I_K_Ca + I_c + I_h) + I_app) / P.C_M;
     c.q = q; c.r = r; c.s = sK; c.ca = Ca; c.icah = I_CaH;
+    I_h_out = I_h;
+    // left to right as the reference: ((((I_CaH + I_sd) + I_ld) + I_K_Ca) + I_c) + I_h
+    return I_CaH + I_sd + I_ld + I_K_Ca;
+}
+
+__device__ __forceinline__ double ode_dend_post(const Model<double> &P, double prevV_dend, double pre, double I_h,
+                                                double I_c, double I_app)
+{
+    double dVd_dt = (-(pre + I_c + I_h) + I_app) / P.C_M;
     return P.DELTA * dVd_dt + prevV_dend;
+}
+
+__device__ __forceinline__ double ode_dend(const Model<double> &P, CellState64 &c, double prevV_dend,
+                                           double prevV_soma, double I_c, double I_app)
+{
+    double I_h;
+    const double pre = ode_dend_pre(P, c, prevV_dend, prevV_soma, I_h);
+    return ode_dend_post(P, prevV_dend, pre, I_h, I_c, I_app);
 }
 
 /* CompSoma ST:560-616 */
@@ -406,6 +445,28 @@ __device__ __forceinline__ float dend(const Model<float> &P, CellState32 &c, flo
                                       float iApp)
 {
     return ode_dend<FM>(P, c, Vd, Vs, I_c, iApp);
+}
+template <bool FM>
+__device__ __forceinline__ double dend_pre(const Model<double> &P, CellState64 &c, double Vd, double Vs, double &I_h)
+{
+    return ode_dend_pre(P, c, Vd, Vs, I_h);
+}
+template <bool FM>
+__device__ __forceinline__ float dend_pre(const Model<float> &P, CellState32 &c, float Vd, float Vs, float &I_h)
+{
+    return ode_dend_pre<FM>(P, c, Vd, Vs, I_h);
+}
+template <bool FM>
+__device__ __forceinline__ double dend_post(const Model<double> &P, double Vd, double pre, double I_h, double I_c,
+                                            double iApp)
+{
+    return ode_dend_post(P, Vd, pre, I_h, I_c, iApp);
+}
+template <bool FM>
+__device__ __forceinline__ float dend_post(const Model<float> &P, float Vd, float pre, float I_h, float I_c,
+                                           float iApp)
+{
+    return ode_dend_post<FM>(P, Vd, pre, I_h, I_c, iApp);
 }
 template <bool FM>
 __device__ __forceinline__ double soma(const Model<double> &P, CellState64 &c, double Vd, double Vs, double Va)
@@ -1127,6 +1188,31 @@ __device__ __forceinline__ void tile_dend_cell(const StepArgs<T> &a, int64_t cel
     a.vd[nxt][cell] = vd;
 }
 
+/* The dendrite up to its voltage, beside the tiles (no I_c needed yet):
+ * gates, Ca2+ and I_CaH in place, the partial current sum and I_h to dpre. */
+template <typename T, bool FM>
+__device__ __forceinline__ void tile_dend_pre_cell(const StepArgs<T> &a, int64_t cell, int64_t t)
+{
+    const int64_t np = a.npad;
+    T *st = a.st;
+    const int cur = (int)(t & 1);
+    typename CellOf<T>::type c;
+    c.q = st[HCURRENT_Q * np + cell];
+    c.r = st[CALCIUM_R * np + cell];
+    c.s = st[POTASSIUM_S * np + cell];
+    c.ca = st[CA2PLUS * np + cell];
+    c.icah = st[I_CAH * np + cell];
+    T I_h;
+    const T pre = dend_pre<FM>(a.m, c, a.vd[cur][cell], a.vs[cur][cell], I_h);
+    st[HCURRENT_Q * np + cell] = c.q;
+    st[CALCIUM_R * np + cell] = c.r;
+    st[POTASSIUM_S * np + cell] = c.s;
+    st[CA2PLUS * np + cell] = c.ca;
+    st[I_CAH * np + cell] = c.icah;
+    a.dpre[cell] = pre;
+    a.dpre[np + cell] = I_h;
+}
+
 template <typename T, bool FM>
 __device__ __forceinline__ void tile_soma_cell(const StepArgs<T> &a, int64_t cell, int64_t t)
 {
@@ -1185,14 +1271,18 @@ __global__ __launch_bounds__(192) void k_tile_ode3(StepArgs<T> a, int64_t k)
     else tile_axon_cell<T, FM>(a, cell, t);
 }
 
-/* Dendrites only (their somata and axons ran beside the tiles), one thread
- * per cell. */
+/* Dendrite voltages (the rest of every compartment ran beside the tiles),
+ * one thread per cell: this step's I_c from the tile partials, then the
+ * voltage from the partial current sum. */
 template <typename T, bool KAHAN, bool FM>
 __global__ __launch_bounds__(kBlock) void k_tile_dend(StepArgs<T> a, int64_t k)
 {
     const int64_t t = (a.d_step ? *a.d_step : 0) + k;
     const int64_t cell = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (cell < a.n) tile_dend_cell<T, KAHAN, FM>(a, cell, t);
+    if (cell >= a.n) return;
+    const int cur = (int)(t & 1), nxt = cur ^ 1;
+    a.vd[nxt][cell] = dend_post<FM>(a.m, a.vd[cur][cell], a.dpre[cell], a.dpre[a.npad + cell],
+                                    tiled_ic<T, KAHAN>(a, cell), iapp_of(a, cell, t));
 }
 
 /* One tile: copy the sender chunk's V_dend to LDS, then each receiver's
@@ -1208,19 +1298,24 @@ __global__ __launch_bounds__(NT, (sizeof(T) == 4 && !HYB) ? (AWC ? 4 : 8) : 1) v
     constexpr int CPW = 64 / L;
     const int64_t t = (a.d_step ? *a.d_step : 0) + k;
     if ((int)blockIdx.x >= a.ntiles) {
-        // workgroups past the tiles: somata (even waves) and axons (odd
-        // waves) of groups of NT/2 cells, which need no gap-junction current,
-        // so they run beside the tiles; workgroup g takes groups g, g + nsa, ...
+        // workgroups past the tiles: somata (even waves), axons and the
+        // dendrites up to their voltage (odd waves) of groups of NT/2 cells,
+        // which need no gap-junction current, so they run beside the tiles;
+        // workgroup g takes groups g, g + nsa, ...
         const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
         for (int64_t g = (int64_t)blockIdx.x - a.ntiles; g * (NT / 2) < a.n; g += a.nsa) {
             const int64_t cell = (g * (NT / 128) + (wv >> 1)) * 64 + (threadIdx.x & 63);
             if (cell < a.n) {
+                // even waves: somata; odd waves: axons, then the dendrites up
+                // to their voltage (about 12 exp each way)
                 if (a.fast_math) {
                     if (wv & 1) tile_axon_cell<T, true>(a, cell, t);
                     else tile_soma_cell<T, true>(a, cell, t);
+                    if (wv & 1) tile_dend_pre_cell<T, true>(a, cell, t);
                 } else {
                     if (wv & 1) tile_axon_cell<T, false>(a, cell, t);
                     else tile_soma_cell<T, false>(a, cell, t);
+                    if (wv & 1) tile_dend_pre_cell<T, false>(a, cell, t);
                 }
             }
         }
