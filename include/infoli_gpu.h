@@ -246,8 +246,9 @@ typedef struct infoli_exec {
     int     lds_kb;          /* staged / tiled: LDS budget per workgroup, KiB */
     int     tile_cells;      /* tiled: receivers per tile (multiple of 64)    */
     int     ode_split;       /* tiled compartment update: 0 one thread per
-                                cell; 1 one wavefront per compartment; 2 somata
-                                and axons beside the tiles, dendrites after   */
+                                cell; 1 one wavefront per compartment; 2 somata,
+                                axons and dendrite gates beside the tiles, the
+                                dendrite voltages after (same results)        */
     int     fast_math;       /* tiled, FP32: hardware exp2 and reciprocal in
                                 the compartment update (0 = libm expf and IEEE
                                 division, the reference's arithmetic)        */

@@ -966,13 +966,13 @@ void default_exec(infoli_ctx *c)
         c->exec.block = 1024;
         c->exec.lanes_per_cell = L;
         c->exec.tile_cells = R;
-        // somata and axons beside the tiles when the tiles need more than
-        // one round of workgroup slots (they then fill the last round), or,
-        // with chunks sized for even rounds, only when every CU holds one
-        // tile and has a slot to spare and the tile rows are short (the
-        // tiles then wait on memory and leave issue slots free); in a single
-        // uneven round, in full even rounds or beside issue-bound tiles they
-        // would only lengthen the launch
+        // somata, axons and dendrite gates beside the tiles when the tiles
+        // need more than one round of workgroup slots (they then fill the
+        // last round), or, with chunks sized for even rounds, only when every
+        // CU holds one tile and has a slot to spare and the tile rows are
+        // short (the tiles then wait on memory and leave issue slots free);
+        // in a single uneven round, in full even rounds or beside issue-bound
+        // tiles they would only lengthen the launch
         const void *fg = hyb_active(c) ? k_tile_gj_hyb_tab[ntidx(1024)][lidx(L)]
                                        : k_tile_gj_tab[c->prec][ntidx(1024)][lidx(L)];
         const size_t lds = (size_t)Ck * c->tsize;

@@ -148,10 +148,10 @@ def test_exec_strategies_agree(mode, mname, L, gather, kb, gname, prec):
 @pytest.mark.parametrize("block", [256, 1024])
 def test_compartment_splits_bitwise_equal(prec, fm, block):
     """The three placements of the compartment update (one thread per cell;
-    one wavefront per compartment; somata and axons beside the tiles,
-    dendrites after) run the same arithmetic on the same start-of-step
-    values, so they agree bit for bit, trace included, with either
-    compartment arithmetic."""
+    one wavefront per compartment; somata, axons and dendrite gates beside
+    the tiles, dendrite voltages after) run the same arithmetic on the same
+    start-of-step values, so they agree bit for bit, trace included, with
+    either compartment arithmetic."""
     say(f"ode_split {prec} fast_math {fm} block {block}")
     n = 2000
     rp, col, w, gcal, iapp = _random_case(n, p=0.05, seed=17)
