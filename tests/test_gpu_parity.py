@@ -447,6 +447,31 @@ def test_partition_batched_equals_separate():
         assert np.abs(o - out[:, a:b]).max() < 1e-11
 
 
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-9), ("fp32k", 2e-4)])
+def test_block_diagonal_default_vs_oracle(prec, tol):
+    """Many independent networks side by side, more receiver blocks than the
+    heuristic's threshold: the default strategy gives every block one tile
+    that stages only its own networks' sender range (1024 lanes per tile,
+    somata, axons and dendrite gates beside the tiles).  Cells differ in
+    g_CaL and Iapp, so the gap-junction currents are not zero; the run
+    matches the oracle."""
+    say(f"block-diagonal default {prec}")
+    wl = W.batched_512(n_nets=560, lo=300, hi=700, steps=60)
+    n = wl["n"]
+    assert n > 512 * 512
+    gcal = I.gen_uniform(41, 2, n, 0.5, 1.1)
+    iapp = I.gen_uniform(41, 3, n, -1.0, 3.0)
+    net = make_net(n, prec, wl["row_ptr"], wl["col"], wl["w"], gcal, iapp)
+    net.set_partition(wl["partition"])
+    e = net.exec
+    assert e["gather"] == I.GATHER_TILED and e["ode_split"] == 2
+    assert e["lanes_per_cell"] * e["tile_cells"] == 1024 or e["tile_cells"] == 256
+    out = net.advance(60, trace_stride=10)
+    net.close()
+    ref, _ = oracle_ref(prec, n, 60, wl["row_ptr"], wl["col"], wl["w"], gcal, iapp, stride=10)
+    assert np.abs(out - ref).max() < tol
+
+
 def test_autotune_restores_state():
     say("autotune")
     n = 5000
