@@ -472,6 +472,51 @@ def test_block_diagonal_default_vs_oracle(prec, tol):
     assert np.abs(out - ref).max() < tol
 
 
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-9), ("fp32k", 2e-4)])
+def test_mixed_block_ranges_vs_oracle(prec, tol):
+    """Small independent networks (whose receiver blocks stage only their own
+    sender range, starting anywhere) next to one large random network
+    (whose blocks take whole chunks) and a few isolated cells, in one
+    context: each tile layout, and both compartment placements, match the
+    oracle."""
+    say(f"mixed block ranges {prec}")
+    rng = np.random.default_rng(57)
+    sizes = [37, 150, 300, 91, 500, 64, 213]
+    rows = []
+    base = 0
+    for m in sizes:
+        rp, col, w = I.gen_symmetric(int(rng.integers(1 << 30)), m, 0.2, 0.02, 0.06)
+        rows.append((rp, col + base, w))
+        base += m
+    big = 3000
+    rp, col, w = I.gen_symmetric(58, big, 0.05, 0.02, 0.06)
+    rows.append((rp, col + base, w))
+    base += big
+    n = base + 5                      # five isolated cells at the end
+    rps, cols, ws, off = [np.zeros(1, np.int64)], [], [], 0
+    for rp, col, w in rows:
+        rps.append(rp[1:] + off)
+        cols.append(col)
+        ws.append(w)
+        off += int(rp[-1])
+    rps.append(np.full(5, off, np.int64))
+    row_ptr = np.concatenate(rps)
+    col = np.concatenate(cols).astype(np.int32)
+    w = np.concatenate(ws)
+    gcal = I.gen_uniform(59, 2, n, 0.5, 1.1)
+    iapp = I.gen_uniform(59, 3, n, -1.0, 3.0)
+    ref, _ = oracle_ref(prec, n, 200, row_ptr, col, w, gcal, iapp, stride=5)
+    for split in (1, 2):
+        # 2 KiB chunks: 512 FP32 / 256 FP64 senders; 256-receiver blocks of the
+        # small networks fit one, those of the large one do not
+        net = make_net(n, prec, row_ptr, col, w, gcal, iapp)
+        net.set_exec(mode=I.EXEC_GRAPH, gather=I.GATHER_TILED, lds_kb=2, tile_cells=256, block=512,
+                     lanes_per_cell=2, ode_split=split, steps_per_launch=50)
+        out = net.advance(200, trace_stride=5)
+        net.close()
+        assert np.abs(out - ref).max() < tol
+
+
 def test_autotune_restores_state():
     say("autotune")
     n = 5000
