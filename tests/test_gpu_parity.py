@@ -568,6 +568,27 @@ def test_sparse_100k_full_size_precision():
     assert err <= 1e-3
 
 
+def test_dense_10k_full_size_precision():
+    """The 10,000-cell dense configuration (about 1,000 junctions per cell,
+    every one of conductance 0.04) at full size: over the first 100 ms
+    FP32+compensated stays within the 1e-3 mV budget of FP64, and the
+    compensated sums stay closer to FP64 than plain FP32 sums do (a junction
+    term with a bias common to all of a cell's terms fails both)."""
+    say("dense_10k full size")
+    wl = W.dense_10k()
+    tr = {}
+    for prec in ("fp64", "fp32k", "fp32"):
+        net = I.Network(wl["n"], precision=prec)
+        W.load_into(net, wl)
+        tr[prec] = net.advance(2000, trace_stride=1)
+        net.close()
+    err_k = np.abs(tr["fp32k"] - tr["fp64"]).max()
+    err_p = np.abs(tr["fp32"] - tr["fp64"]).max()
+    say(f"  max |dV| vs FP64 over 100 ms: compensated {err_k:.3e} mV, plain {err_p:.3e} mV")
+    assert err_k <= 1e-3
+    assert err_k < err_p
+
+
 @pytest.mark.parametrize("prec", ["fp32", "fp32k"])
 def test_fast_math_compartments(prec):
     """Hardware exp2 / reciprocal in the compartment update (tiled path):
