@@ -512,13 +512,13 @@ template <> struct CellOf<double> { using type = CellState64; };
 
 /* w * f(V) * V with f(V) = 0.8 exp(-V^2/100) + 0.2 (OMP:514-517, ST:549-552).
  * FP32: exp(-V^2/100) = exp2(V^2 * (-log2(e)/100)) on the hardware exp2
- * unit (one multiply, one v_exp_f32); FP64: libm-accurate exp. */
-/* 0.8 e + 0.2 is one fused multiply-add, then times w and V.  Of the forms
- * measured against FP64 on the 10,000-cell dense network this one keeps the
- * compensated sums closest (7.2e-4 mV over 100 ms, against 8.2e-4 with a
- * separate multiply and add); folding a constant w into the constants
- * (0.8 w, 0.2 w) gives every term of a cell the same rounding bias, which
- * the compensated sum then accumulates faithfully (1.3e-3 mV). */
+ * unit (one multiply, one v_exp_f32), and 0.8 e + 0.2 as one fused
+ * multiply-add, then times w and V.  Of the forms measured against FP64 on
+ * the 10,000-cell dense network this one keeps the compensated sums closest
+ * (7.2e-4 mV over 100 ms, against 8.2e-4 with a separate multiply and add);
+ * folding a constant w into the constants (0.8 w, 0.2 w) gives every term
+ * of a cell the same rounding bias, which the compensated sum then
+ * accumulates faithfully (1.3e-3 mV).  FP64: libm-accurate exp. */
 __device__ __forceinline__ float gj_term(float vi, float vj, float w)
 {
     const float K = -0.014426950408889634f; /* -log2(e) / 100 */
@@ -527,7 +527,6 @@ __device__ __forceinline__ float gj_term(float vi, float vj, float w)
     float f = __builtin_fmaf(0.8f, e, 0.2f);
     return w * f * V;
 }
-
 
 __device__ __forceinline__ double gj_term(double vi, double vj, double w)
 {
@@ -1281,8 +1280,9 @@ __global__ __launch_bounds__(kBlock) void k_tile_dend(StepArgs<T> a, int64_t k)
 /* One tile: copy the sender chunk's V_dend to LDS, then each receiver's
  * junctions from that chunk (L lanes per receiver, list order) into a
  * partial (sum, compensation) pair.  AWC: every tile has a constant
- * conductance (FP32 only), and the rows prefetch their next ids, which
- * takes more registers (four waves per SIMD rather than eight). */
+ * conductance (FP32 only), and the rows prefetch their next ids; its launch
+ * bounds leave room for the extra registers (four waves per SIMD rather
+ * than eight; it compiles to 60 VGPRs, still eight). */
 template <typename T, int L, bool KAHAN, int NT, bool HYB = false, bool AWC = false>
 __global__ __launch_bounds__(NT, (sizeof(T) == 4 && !HYB) ? (AWC ? 4 : 8) : 1) void k_tile_gj(StepArgs<T> a, int64_t k)
 {
