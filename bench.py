@@ -27,14 +27,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-GATHER_NAMES = {0: "direct", 1: "lds", 2: "staged", 3: "tiled"}
+GATHER_NAMES = {0: "direct", 1: "lds", 2: "staged", 3: "tiled", 4: "sweep"}
 TILED_KERNELS = {0: "k_tile_gj+k_tile_ode", 1: "k_tile_gj+k_tile_ode3", 2: "k_tile_gj+k_tile_dend"}
+SWEEP_KERNELS = {0: "k_tile_sweep+k_tile_ode", 1: "k_tile_sweep+k_tile_ode3"}
 
 
 def kernel_name(exec_cfg):
     """The kernels of one step under this execution strategy."""
     if exec_cfg["gather"] == 3:
         return TILED_KERNELS[exec_cfg["ode_split"]]
+    if exec_cfg["gather"] == 4:
+        return SWEEP_KERNELS[exec_cfg["ode_split"]]
     return "k_step"
 
 
@@ -156,7 +159,9 @@ def main():
     ap.add_argument("--stride", type=int, default=100)
     ap.add_argument("--mode", default="auto", choices=["auto", "graph", "persistent", "step", "tune"])
     ap.add_argument("--lanes", type=int, default=0)
-    ap.add_argument("--gather", default="auto", choices=["auto", "direct", "lds", "staged", "tiled"])
+    ap.add_argument("--gather", default="auto", choices=["auto", "direct", "lds", "staged", "tiled", "sweep"])
+    ap.add_argument("--ode-split", type=int, default=-1,
+                    help="compartment placement with the tiled gathers (0, 1, 2; default: heuristic)")
     ap.add_argument("--lds-kb", type=int, default=0)
     ap.add_argument("--tile-cells", type=int, default=0)
     ap.add_argument("--steps-per-launch", type=int, default=0,
@@ -191,8 +196,9 @@ def main():
         spl = args.steps_per_launch or (500 if wl.get("partition") is not None else 0)
         if spl:
             net.set_exec(steps_per_launch=spl)
-        if args.gather != "auto":
-            net.set_exec(gather={"direct": 0, "lds": 1, "staged": 2, "tiled": 3}[args.gather])
+        if args.gather != "auto" or args.ode_split >= 0:
+            g = {"direct": 0, "lds": 1, "staged": 2, "tiled": 3, "sweep": 4}.get(args.gather)
+            net.set_exec(gather=g, ode_split=args.ode_split if args.ode_split >= 0 else None)
         if args.mode == "graph":
             net.set_exec(mode=I.EXEC_GRAPH)
         elif args.mode == "persistent":

@@ -235,19 +235,29 @@ int infoli_gap_currents(infoli_t *ctx, double *i_c);
 #define INFOLI_GATHER_TILED     3  /* receiver block x sender chunk tiles: the
                                       chunk's V_dend in LDS, per-tile partial
                                       sums merged by the compartment kernel     */
+#define INFOLI_GATHER_SWEEP     4  /* the same tiles, one 1024-thread workgroup
+                                      per receiver block walking its chunks in
+                                      order, the partial sums merged in LDS
+                                      (same results as TILED at equal shapes)  */
 
 typedef struct infoli_exec {
     int     mode;            /* INFOLI_EXEC_*                                 */
     int     lanes_per_cell;  /* 1,2,4,8,16: lanes sharing one neighbour list  */
-    int     block;           /* threads per workgroup (256)                   */
+    int     block;           /* threads per workgroup (256; tiled 256, 512 or
+                                1024; sweep 1024)                            */
     int     steps_per_launch;/* graph length / persistent steps per launch    */
     int     gather;          /* INFOLI_GATHER_*                               */
-    int     lds_kb;          /* staged / tiled: LDS budget per workgroup, KiB */
-    int     tile_cells;      /* tiled: receivers per tile (multiple of 64)    */
+    int     lds_kb;          /* staged / tiled / sweep: LDS budget of the
+                                gather list or chunk, KiB (the sweep's R sums
+                                take 4 or 8 bytes each beside it, 160 KiB in
+                                all)                                         */
+    int     tile_cells;      /* tiled / sweep: receivers per tile (multiple
+                                of 64)                                       */
     int     ode_split;       /* tiled compartment update: 0 one thread per
                                 cell; 1 one wavefront per compartment; 2 somata,
                                 axons and dendrite gates beside the tiles, the
-                                dendrite voltages after (same results)        */
+                                dendrite voltages after (same results; not
+                                with the sweep)                              */
     int     fast_math;       /* tiled, FP32: hardware exp2 and reciprocal in
                                 the compartment update (0 = libm expf and IEEE
                                 division, the reference's arithmetic)        */

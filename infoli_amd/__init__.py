@@ -18,6 +18,7 @@ from .network import (  # noqa: F401
     GATHER_LDS,
     GATHER_STAGED,
     GATHER_TILED,
+    GATHER_SWEEP,
     VARS,
     InfoliError,
     Network,

@@ -43,6 +43,7 @@ namespace {
 constexpr size_t kTraceRingBytes = size_t(1) << 30;   // device trace ring budget
 constexpr int64_t kTracePieceSteps = 1000;             // steps per trace piece (copy overlaps the next)
 constexpr size_t kVldsMaxBytes = 96 * 1024;           // V_dend staged in LDS up to this size
+constexpr size_t kLdsBytes = 160 * 1024;              // LDS of a CU (gfx950)
 constexpr int kDefaultLdsKb = 76;                     // staged gather: two workgroups per CU
 
 int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
@@ -92,6 +93,7 @@ struct infoli_ctx {
     bool tl_allwc = false;  // every tile has a constant conductance
     TileDesc *d_tiles = nullptr;
     int32_t *d_rb_tile0 = nullptr;
+    int32_t *d_rb_one = nullptr;   // 0, 1, ..., nrb: one partial per block when swept
     int64_t *d_tsp = nullptr;
     uint16_t *d_rlist = nullptr, *d_rdeg = nullptr, *d_tcol = nullptr;
     void *d_tw = nullptr, *d_part = nullptr;
@@ -187,6 +189,9 @@ bool stream_exceeds_cache(const infoli_ctx *c)
     return (double)c->col.size() * (double)(4 + c->tsize) > 128.0 * 1024 * 1024;
 }
 
+/* the gathers built on the 2D-tiled layout */
+bool tiled_layout(const infoli_exec &x) { return x.gather == GATHER_TILED || x.gather == GATHER_SWEEP; }
+
 /* some cell sums FP64 gap-junction terms (FP32 compensated, degree budget) */
 bool hyb_active(const infoli_ctx *c)
 {
@@ -236,9 +241,10 @@ StepArgs<T> make_args(const infoli_ctx *c, int64_t stride)
     } else {
         a.nblocks = c->npad / kBlock;
     }
-    if (c->exec.gather == GATHER_TILED) {
+    if (tiled_layout(c->exec)) {
         a.tiles = c->d_tiles;
         a.rb_tile0 = c->d_rb_tile0;
+        a.rb_part0 = c->exec.gather == GATHER_SWEEP ? c->d_rb_one : c->d_rb_tile0;
         a.tsp = c->d_tsp;
         a.rlist = c->d_rlist;
         a.rdeg = c->d_rdeg;
@@ -329,6 +335,14 @@ const void *const k_tile_gj_hyb_tab[3][5] = {
     INFOLI_L_TABLE(k_tile_gj, float, true, 1024, true),
 };
 const void *const k_gap_hyb_tab[5] = INFOLI_L_TABLE(k_gap_currents, float, true, true);
+/* one workgroup of 1024 threads per receiver block: [precision][L], and
+ * FP32 compensated with FP64 sums for high-degree cells: [L] */
+const void *const k_tile_sweep_tab[3][5] = {
+    INFOLI_L_TABLE(k_tile_sweep, double, false, 1024),
+    INFOLI_L_TABLE(k_tile_sweep, float, false, 1024),
+    INFOLI_L_TABLE(k_tile_sweep, float, true, 1024),
+};
+const void *const k_tile_sweep_hyb_tab[5] = INFOLI_L_TABLE(k_tile_sweep, float, true, 1024, true);
 /* [fast math][precision] */
 const void *const k_tile_dend_tab[2][3] = {
     {(const void *)k_tile_dend<double, false, false>, (const void *)k_tile_dend<float, false, false>,
@@ -343,6 +357,22 @@ const void *const k_gap_tab[3][5] = {
 };
 
 bool is_f32(const infoli_ctx *c) { return c->prec != INFOLI_FP64; }
+
+/* bytes of one partial (sum, compensation) pair */
+size_t part_bytes(const infoli_ctx *c) { return c->prec == INFOLI_FP32_KAHAN ? 8 : c->tsize; }
+
+/* the sweep's LDS: one accumulator per receiver of the block, then the chunk */
+size_t sweep_lds_bytes(const infoli_ctx *c, int R, int64_t C)
+{
+    return (size_t)R * part_bytes(c) + (size_t)C * c->tsize;
+}
+
+/* the largest chunk budget (KiB) that fits beside R receivers' sums */
+int sweep_max_kb(const infoli_ctx *c, int R)
+{
+    const int64_t room = (int64_t)kLdsBytes - (int64_t)R * (int64_t)part_bytes(c);
+    return (int)std::min<int64_t>(156, room / 1024);
+}
 
 size_t dyn_lds_bytes(const infoli_ctx *c)
 {
@@ -368,6 +398,13 @@ const void *tile_kernel(const infoli_ctx *c)
     return k_tile_gj_tab[c->prec][ntidx(c->exec.block)][li];
 }
 
+/* the sweeping kernel of the context's precision and lanes */
+const void *sweep_kernel(const infoli_ctx *c)
+{
+    const int li = lidx(c->exec.lanes_per_cell);
+    return hyb_active(c) ? k_tile_sweep_hyb_tab[li] : k_tile_sweep_tab[c->prec][li];
+}
+
 /* raise the dynamic-LDS limit of a kernel above 64 KiB when needed */
 void allow_dyn_lds(const void *fn, size_t bytes)
 {
@@ -382,24 +419,32 @@ int64_t n_blocks(const infoli_ctx *c)
 
 int launch_step(infoli_ctx *c, const void *args, int64_t k)
 {
-    if (c->exec.gather == GATHER_TILED) {
-        const void *fg = tile_kernel(c);
+    if (tiled_layout(c->exec)) {
         const int fm = c->exec.fast_math ? 1 : 0;
         const void *fo = k_tile_ode_tab[fm][c->prec];
         int64_t kk = k;
         void *kargs[] = {const_cast<void *>(args), &kk};
-        const size_t lds = (size_t)c->tl_C * c->tsize;
-        allow_dyn_lds(fg, lds);
-        if (c->exec.ode_split == 2) {
-            // tiles, then the soma/axon workgroups in the same launch; the
-            // dendrites (which need the tiles' partials) after
-            HIPCHK(hipLaunchKernel(fg, dim3((unsigned)(c->tl_ntiles + c->tl_nsa)), dim3(c->exec.block), kargs, lds,
-                                   c->stream));
-            HIPCHK(hipLaunchKernel(k_tile_dend_tab[fm][c->prec], dim3((unsigned)(c->npad / kBlock)), dim3(kBlock), kargs,
-                                   0, c->stream));
-            return INFOLI_OK;
+        if (c->exec.gather == GATHER_SWEEP) {
+            // one workgroup per receiver block, then the compartments
+            const void *fs = sweep_kernel(c);
+            const size_t lds = sweep_lds_bytes(c, c->tiled_R, c->tl_C);
+            allow_dyn_lds(fs, lds);
+            HIPCHK(hipLaunchKernel(fs, dim3((unsigned)c->tl_nrb), dim3(1024), kargs, lds, c->stream));
+        } else {
+            const void *fg = tile_kernel(c);
+            const size_t lds = (size_t)c->tl_C * c->tsize;
+            allow_dyn_lds(fg, lds);
+            if (c->exec.ode_split == 2) {
+                // tiles, then the soma/axon workgroups in the same launch;
+                // the dendrites (which need the tiles' partials) after
+                HIPCHK(hipLaunchKernel(fg, dim3((unsigned)(c->tl_ntiles + c->tl_nsa)), dim3(c->exec.block), kargs,
+                                       lds, c->stream));
+                HIPCHK(hipLaunchKernel(k_tile_dend_tab[fm][c->prec], dim3((unsigned)(c->npad / kBlock)), dim3(kBlock),
+                                       kargs, 0, c->stream));
+                return INFOLI_OK;
+            }
+            HIPCHK(hipLaunchKernel(fg, dim3((unsigned)c->tl_ntiles), dim3(c->exec.block), kargs, lds, c->stream));
         }
-        HIPCHK(hipLaunchKernel(fg, dim3((unsigned)c->tl_ntiles), dim3(c->exec.block), kargs, lds, c->stream));
         if (c->exec.ode_split)
             HIPCHK(hipLaunchKernel(k_tile_ode3_tab[fm][c->prec], dim3((unsigned)(c->npad / 64)), dim3(192), kargs, 0,
                                    c->stream));
@@ -514,13 +559,13 @@ void free_staged(infoli_ctx *c)
  * list; and per cell its LDS row start. */
 void free_tiled(infoli_ctx *c)
 {
-    void *bufs[] = {c->d_tiles, c->d_rb_tile0, c->d_tsp, c->d_rlist, c->d_rdeg, c->d_tcol, c->d_tw, c->d_part,
-                    c->d_dpre};
+    void *bufs[] = {c->d_tiles, c->d_rb_tile0, c->d_rb_one, c->d_tsp, c->d_rlist, c->d_rdeg, c->d_tcol, c->d_tw,
+                    c->d_part, c->d_dpre};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     c->d_dpre = nullptr;
     c->d_tiles = nullptr;
-    c->d_rb_tile0 = nullptr;
+    c->d_rb_tile0 = c->d_rb_one = nullptr;
     c->d_tsp = nullptr;
     c->d_rlist = c->d_rdeg = c->d_tcol = nullptr;
     c->d_tw = c->d_part = nullptr;
@@ -642,18 +687,23 @@ int upload_tiled(infoli_ctx *c, int L, int R, int kb)
         std::fill(rbt0.begin(), rbt0.end(), 0);
     }
     free_tiled(c);
-    const size_t psz = is_f32(c) && c->prec == INFOLI_FP32_KAHAN ? 8 : c->tsize;
+    const size_t psz = part_bytes(c);
+    std::vector<int32_t> rb_one(nrb + 1);
+    for (int64_t rb = 0; rb <= nrb; rb++) rb_one[rb] = (int32_t)rb;
     HIPCHK(hipMalloc(&c->d_tiles, sizeof(TileDesc) * tiles.size()));
     HIPCHK(hipMalloc(&c->d_rb_tile0, sizeof(int32_t) * rbt0.size()));
+    HIPCHK(hipMalloc(&c->d_rb_one, sizeof(int32_t) * rb_one.size()));
     HIPCHK(hipMalloc(&c->d_tsp, sizeof(int64_t) * tsp.size()));
     HIPCHK(hipMalloc(&c->d_rlist, sizeof(uint16_t) * std::max<size_t>(rlist.size(), 1)));
     HIPCHK(hipMalloc(&c->d_rdeg, sizeof(uint16_t) * std::max<size_t>(rdeg.size(), 1)));
     HIPCHK(hipMalloc(&c->d_tcol, sizeof(uint16_t) * std::max<size_t>(tcol.size(), 1)));
     HIPCHK(hipMalloc(&c->d_tw, sizeof(T) * std::max<size_t>(tw.size(), 1)));
-    HIPCHK(hipMalloc(&c->d_part, psz * tiles.size() * (size_t)R));
+    // partials: one per tile, or one per block when swept
+    HIPCHK(hipMalloc(&c->d_part, psz * std::max<size_t>(tiles.size(), (size_t)nrb) * (size_t)R));
     HIPCHK(hipMalloc(&c->d_dpre, 2 * (size_t)c->npad * c->tsize));
     HIPCHK(hipMemcpy(c->d_tiles, tiles.data(), sizeof(TileDesc) * tiles.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_rb_tile0, rbt0.data(), sizeof(int32_t) * rbt0.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_rb_one, rb_one.data(), sizeof(int32_t) * rb_one.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_tsp, tsp.data(), sizeof(int64_t) * tsp.size(), hipMemcpyHostToDevice));
     if (!rlist.empty()) {
         HIPCHK(hipMemcpy(c->d_rlist, rlist.data(), sizeof(uint16_t) * rlist.size(), hipMemcpyHostToDevice));
@@ -782,7 +832,7 @@ int ensure_layout(infoli_ctx *c)
         HIPCHK(hipStreamSynchronize(c->stream));
         RCHK(upload_staged(c, c->exec.lds_kb));
     }
-    if (c->exec.gather == GATHER_TILED &&
+    if (tiled_layout(c->exec) &&
         (!c->tiled_valid || c->tiled_kb != c->exec.lds_kb || c->tiled_L != L || c->tiled_R != c->exec.tile_cells)) {
         HIPCHK(hipSetDevice(c->device));
         HIPCHK(hipStreamSynchronize(c->stream));
@@ -984,6 +1034,30 @@ void default_exec(infoli_ctx *c)
         const bool spare = even_tiles && per_cu >= 2 && ntiles <= (double)c->num_cus && dc < 64.0;
         c->exec.ode_split = (even_tiles ? spare : ntiles > (double)per_cu * c->num_cus) ? 2 : 1;
         return;
+    }
+    // swept tiles for FP32 lists beyond the Infinity Cache with at least 45
+    // junctions per cell: one 1024-thread workgroup per receiver block sums
+    // the block's tiles in LDS.  A block's sums and chunk fill a CU's LDS, so
+    // the blocks number at most the CUs (R = n / CUs, rounded up to 64), and
+    // the chunk takes the LDS the sums leave.  The time goes to per-tile
+    // latency, which grows with R, so R stays at most 4096.  Measured (FP32
+    // compensated, us per step, direct -> swept): 1M cells of degree 50
+    // 372 -> 305, of degree 100 700 -> 354, 500k of degree 50 184 -> 110;
+    // the direct gather stays ahead at degree 40 and below (1M cells:
+    // 307 -> 319 at 40, 177 -> 280 at 20), for 2M cells (1006 -> 1125 at
+    // R 7872) and in FP64 (573 -> 669 at 1M, degree 50)
+    if (is_f32(c) && stream_exceeds_cache(c) && mean_deg >= 45.0) {
+        const int64_t R = round_up((c->n + c->num_cus - 1) / std::max(c->num_cus, 1), 64);
+        const int kb = R <= 4096 ? sweep_max_kb(c, (int)R) : 0;
+        if (kb >= 16) {
+            c->exec.gather = GATHER_SWEEP;
+            c->exec.lanes_per_cell = 1;
+            c->exec.block = 1024;
+            c->exec.tile_cells = (int)R;
+            c->exec.lds_kb = kb;
+            c->exec.ode_split = 1;
+            return;
+        }
     }
     // staged gather when a 64-cell group fits and the lists stay in the
     // Infinity Cache; beyond it both stream from HBM and the direct gather,
@@ -1215,7 +1289,7 @@ extern "C" int infoli_set_fp64_degree(infoli_t *c, int64_t min_degree)
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->fp64_deg = min_degree;
-    if (hyb_active(c) && c->exec.gather != GATHER_TILED) {
+    if (hyb_active(c) && !tiled_layout(c->exec)) {
         const int keep_mode = c->exec.mode;
         default_exec(c);
         if (keep_mode != INFOLI_EXEC_PERSISTENT) c->exec.mode = keep_mode;
@@ -1471,31 +1545,38 @@ int check_exec(const infoli_ctx *c, infoli_exec &x)
         return infoli_fail(INFOLI_ERR_ARG, "infoli_set_exec: mode %d not available", x.mode);
     if (lidx(x.lanes_per_cell) < 0)
         return infoli_fail(INFOLI_ERR_ARG, "lanes_per_cell %d not in {1,2,4,8,16}", x.lanes_per_cell);
-    if (x.gather != GATHER_TILED) x.block = kBlock;   // the fused step kernel has one block size
+    if (x.gather == GATHER_SWEEP) x.block = 1024;       // one workgroup of 1024 threads per receiver block
+    else if (x.gather != GATHER_TILED) x.block = kBlock;   // the fused step kernel has one block size
     if (x.block != 256 && x.block != 512 && x.block != 1024)
         return infoli_fail(INFOLI_ERR_ARG, "block %d: the tiled gather takes 256, 512 or 1024", x.block);
     if (x.steps_per_launch < 1 || x.steps_per_launch > 100000)
         return infoli_fail(INFOLI_ERR_ARG, "steps_per_launch %d out of range", x.steps_per_launch);
-    if (x.gather < GATHER_DIRECT || x.gather > GATHER_TILED)
+    if (x.gather < GATHER_DIRECT || x.gather > GATHER_SWEEP)
         return infoli_fail(INFOLI_ERR_ARG, "infoli_set_exec: gather %d unknown", x.gather);
-    if (x.gather == GATHER_TILED && (x.tile_cells < 64 || x.tile_cells > 16384 || x.tile_cells % 64))
+    if (tiled_layout(x) && (x.tile_cells < 64 || x.tile_cells > 16384 || x.tile_cells % 64))
         return infoli_fail(INFOLI_ERR_ARG, "tile_cells %d: must be a multiple of 64 in [64, 16384]",
                            x.tile_cells);
-    if (x.fast_math && x.gather != GATHER_TILED)
-        return infoli_fail(INFOLI_ERR_ARG, "fast_math is available with the tiled gather only");
+    if (x.fast_math && !tiled_layout(x))
+        return infoli_fail(INFOLI_ERR_ARG, "fast_math is available with the tiled gathers only");
     if (x.ode_split < 0 || x.ode_split > 2)
         return infoli_fail(INFOLI_ERR_ARG, "ode_split %d not in {0, 1, 2}", x.ode_split);
-    if (hyb_active(c) && x.gather != GATHER_TILED)
+    if (hyb_active(c) && !tiled_layout(x))
         return infoli_fail(INFOLI_ERR_ARG, "cells with at least %lld neighbours sum FP64 gap-junction terms, "
-                                           "which the tiled gather implements; set the FP64 degree to 0 "
+                                           "which the tiled gathers implement; set the FP64 degree to 0 "
                                            "for another gather", (long long)c->fp64_deg);
-    if (x.gather == GATHER_TILED && x.mode == INFOLI_EXEC_PERSISTENT)
-        return infoli_fail(INFOLI_ERR_ARG, "the tiled gather runs as two kernels per step; "
+    if (tiled_layout(x) && x.mode == INFOLI_EXEC_PERSISTENT)
+        return infoli_fail(INFOLI_ERR_ARG, "the tiled gathers run as two kernels per step; "
                                            "use the step or graph launch mode");
+    if (x.gather == GATHER_SWEEP && x.ode_split == 2)
+        return infoli_fail(INFOLI_ERR_ARG, "the swept tiles fill a CU's LDS, so no somata and axons run beside "
+                                           "them: ode_split 0 or 1");
     if (x.gather == GATHER_VLDS && (size_t)c->n * c->tsize > kVldsMaxBytes)
         return infoli_fail(INFOLI_ERR_ARG, "gather LDS: %lld cells do not fit the LDS budget", (long long)c->n);
     if (x.lds_kb < 1 || x.lds_kb > 156)
         return infoli_fail(INFOLI_ERR_ARG, "lds_kb %d outside [1, 156]", x.lds_kb);
+    if (x.gather == GATHER_SWEEP && sweep_lds_bytes(c, x.tile_cells, tiled_chunk(c, x.lds_kb)) > kLdsBytes)
+        return infoli_fail(INFOLI_ERR_ARG, "swept tiles: %d receivers' sums plus a %d KiB chunk exceed the "
+                                           "160 KiB of LDS", x.tile_cells, x.lds_kb);
     if (x.gather == GATHER_STAGED && !max_block_entries_fit(c, x.lds_kb))
         return infoli_fail(INFOLI_ERR_ARG, "staged gather: a 64-cell group exceeds the %d KiB LDS budget",
                            x.lds_kb);
@@ -1640,7 +1721,7 @@ int run_steps(infoli_ctx *c, StepArgs<T> a, int64_t t, int64_t count)
         return INFOLI_OK;
     }
     // persistent
-    if (c->exec.gather == GATHER_TILED)
+    if (tiled_layout(c->exec))
         return infoli_fail(INFOLI_ERR_ARG, "tiled gather has no persistent form");
     int grid = persist_grid(c);
     if (grid <= 0) return grid < 0 ? grid : infoli_fail(INFOLI_ERR_HIP, "no persistent grid");
@@ -1864,8 +1945,8 @@ extern "C" int infoli_autotune(infoli_t *c, int64_t trial_steps, infoli_exec *ch
 
     // candidates: the heuristic's choice first, then tiled shapes around it
     // (lanes, receivers per tile and chunk size halved and doubled, both
-    // compartment placements), then the other gathers unless some cell
-    // needs the FP64 sums only the tiled gather has
+    // compartment placements), swept tiles, then the other gathers unless
+    // some cell needs the FP64 sums only the tiled gathers have
     std::vector<infoli_exec> cands;
     const int spl = (int)std::min<int64_t>(trial_steps, 500);
     default_exec(c);
@@ -1888,6 +1969,24 @@ extern "C" int infoli_autotune(infoli_t *c, int64_t trial_steps, infoli_exec *ch
                     const infoli_exec e{INFOLI_EXEC_GRAPH, L, 1024, spl, GATHER_TILED, kb, R, split, 0};
                     if (memcmp(&e, &h, sizeof e) != 0) cands.push_back(e);
                 }
+            }
+        }
+    }
+    // swept tiles, one lane per receiver, about one block per CU (and half
+    // and twice that many) with the largest chunk that fits beside the
+    // blocks' sums, for networks of at least 16 such blocks
+    {
+        const int Rs0 = h.gather == GATHER_SWEEP
+                            ? h.tile_cells
+                            : (int)std::min<int64_t>(16384, round_up((c->n + c->num_cus - 1) /
+                                                                         std::max(c->num_cus, 1), 64));
+        for (int L : {1}) {
+            for (int R : {Rs0 / 2, Rs0, Rs0 * 2}) {
+                if (R < 64 || R > 16384 || R % 64 || c->n < (int64_t)16 * R) continue;
+                const int kb = sweep_max_kb(c, R);
+                if (kb < 1) continue;
+                const infoli_exec e{INFOLI_EXEC_GRAPH, L, 1024, spl, GATHER_SWEEP, kb, R, 1, 0};
+                if (memcmp(&e, &h, sizeof e) != 0) cands.push_back(e);
             }
         }
     }

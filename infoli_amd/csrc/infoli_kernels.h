@@ -18,6 +18,8 @@
  *    into per-tile partial sums, and a compartment kernel (k_tile_ode3,
  *    k_tile_ode or k_tile_dend) merges them in chunk order and updates the
  *    cells; somata and axons can also run in k_tile_gj beside the tiles.
+ *    k_tile_sweep walks the same tiles with one workgroup per receiver
+ *    block and merges the partials in LDS instead.
  * Each row's terms are summed in list (ascending sender) order in every path.
  */
 #pragma once
@@ -95,12 +97,15 @@ struct StepArgs {
     // 2D-tiled gap junctions (receiver block x sender chunk)
     const TileDesc *tiles;     // tiles, receiver-block major, chunks ascending
     const int32_t *rb_tile0;   // first tile of each receiver block, nrb + 1
+    const int32_t *rb_part0;   // first partial of each receiver block, nrb + 1: rb_tile0,
+                               // or rb itself when one workgroup sweeps a block's tiles
     const int64_t *tsp;        // per tile slice: first entry (64 entries per column)
     const uint16_t *rlist;     // per tile: receivers (local to the block), in-chunk degree descending
     const uint16_t *rdeg;      // in-chunk degree of each listed receiver
     const uint16_t *tcol;      // sender index local to the chunk
     const T *tw;               // conductance
-    void *part;                // partial sums, [tile][receiver block size] (sum, comp) pairs
+    void *part;                // partial sums, [tile][receiver block size] (sum, comp) pairs,
+                               // or [block][receiver block size] when swept
     int32_t fp64_deg;          // FP32 compensated: cells with at least this many
                                // neighbours sum FP64 terms (0 = none)
     int32_t ntiles;            // tiles (workgroups past them run somata and axons)
@@ -118,8 +123,10 @@ enum Gather {
     GATHER_DIRECT = 0,   // random loads from the V_dend array (L2-resident)
     GATHER_VLDS = 1,     // the whole V_dend array staged in LDS (small networks)
     GATHER_STAGED = 2,   // per block: senders sorted, copied to LDS, then read in row order
-    GATHER_TILED = 3     // receiver block x sender chunk tiles, chunk of V_dend in LDS,
+    GATHER_TILED = 3,    // receiver block x sender chunk tiles, chunk of V_dend in LDS,
                          // partial sums merged by the compartment kernel
+    GATHER_SWEEP = 4     // the tiled layout, one workgroup per receiver block walking its
+                         // chunks, the partial sums merged in LDS
 };
 
 struct GridBarrier {
@@ -1002,8 +1009,11 @@ __device__ __forceinline__ X ld_off(const X *__restrict__ p, uint32_t byte_off)
 /* One lane's share of a tile row: entries at irow/wrow[(kk*64) + loff],
  * irow and wrow uniform across the wavefront, W the slice width.  Same
  * arithmetic and order as row_terms_idx; WC: every junction has
- * conductance wc. */
-template <typename T, bool KAHAN, int L, bool WC, bool HYB>
+ * conductance wc.  TAG gives each calling kernel its own instantiation:
+ * the batch lambda is inlined by the compiler's cost model, which counts
+ * its call sites, so a caller added elsewhere would otherwise move the
+ * register allocation of k_tile_gj. */
+template <typename T, bool KAHAN, int L, bool WC, bool HYB, int TAG = 0>
 __device__ __forceinline__ void tile_row_terms(Acc<T, KAHAN> &acc, double &accd, bool wide, T vi, int d, int l,
                                                int W, int nvm, int kstart, int loff,
                                                const uint16_t *__restrict__ irow, const T *__restrict__ wrow, T wc,
@@ -1116,14 +1126,15 @@ __device__ __forceinline__ void part_merge(const void *p, int64_t i, Acc<T, KAHA
 }
 
 /* I_c of a cell: its partials over the tiles of its receiver block, merged
- * in ascending sender-chunk order. */
+ * in ascending sender-chunk order (one partial, already merged, when the
+ * block's tiles were swept by one workgroup). */
 template <typename T, bool KAHAN>
 __device__ __forceinline__ T tiled_ic(const StepArgs<T> &a, int64_t cell)
 {
     using P = typename PartOf<T, KAHAN>::type;
     const P *__restrict__ part = reinterpret_cast<const P *>(a.part);
     const int64_t rb = cell / a.rcells, rloc = cell - rb * a.rcells;
-    const int32_t t0 = a.rb_tile0[rb], t1 = a.rb_tile0[rb + 1];   // uniform per wavefront
+    const int32_t t0 = a.rb_part0[rb], t1 = a.rb_part0[rb + 1];   // uniform per wavefront
     Acc<T, KAHAN> acc;
     constexpr int B = 12;   // partials in flight: one batch for up to 12 sender chunks
     for (int32_t tt = t0; tt < t1; tt += B) {
@@ -1279,43 +1290,16 @@ __global__ __launch_bounds__(kBlock) void k_tile_dend(StepArgs<T> a, int64_t k)
 
 /* One tile: copy the sender chunk's V_dend to LDS, then each receiver's
  * junctions from that chunk (L lanes per receiver, list order) into a
- * partial (sum, compensation) pair.  AWC: every tile has a constant
- * conductance (FP32 only), and the rows prefetch their next ids; its launch
- * bounds leave room for the extra registers (four waves per SIMD rather
- * than eight; it compiles to 60 VGPRs, still eight). */
-template <typename T, int L, bool KAHAN, int NT, bool HYB = false, bool AWC = false>
-__global__ __launch_bounds__(NT, (sizeof(T) == 4 && !HYB) ? (AWC ? 4 : 8) : 1) void k_tile_gj(StepArgs<T> a, int64_t k)
+ * partial (sum, compensation) pair, which the receiver's first lane hands to
+ * sink(receiver within the block, pair).  Every receiver of the block is
+ * listed once per tile, so the sink sees each receiver once.  AWC: every
+ * tile has a constant conductance (FP32 only), and the rows prefetch their
+ * next ids. */
+template <typename T, int L, bool KAHAN, int NT, bool HYB, bool AWC, typename Sink>
+__device__ __forceinline__ void tile_rows(const StepArgs<T> &a, const TileDesc td, const T *__restrict__ vin,
+                                          T *__restrict__ vch, Sink &&sink)
 {
-    extern __shared__ __align__(16) unsigned char dyn_lds[];
-    T *vch = reinterpret_cast<T *>(dyn_lds);
     constexpr int CPW = 64 / L;
-    const int64_t t = (a.d_step ? *a.d_step : 0) + k;
-    if ((int)blockIdx.x >= a.ntiles) {
-        // workgroups past the tiles: somata (even waves), axons and the
-        // dendrites up to their voltage (odd waves) of groups of NT/2 cells,
-        // which need no gap-junction current, so they run beside the tiles;
-        // workgroup g takes groups g, g + nsa, ...
-        const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
-        for (int64_t g = (int64_t)blockIdx.x - a.ntiles; g * (NT / 2) < a.n; g += a.nsa) {
-            const int64_t cell = (g * (NT / 128) + (wv >> 1)) * 64 + (threadIdx.x & 63);
-            if (cell < a.n) {
-                // even waves: somata; odd waves: axons, then the dendrites up
-                // to their voltage (about 12 exp each way)
-                if (a.fast_math) {
-                    if (wv & 1) tile_axon_cell<T, true>(a, cell, t);
-                    else tile_soma_cell<T, true>(a, cell, t);
-                    if (wv & 1) tile_dend_pre_cell<T, true>(a, cell, t);
-                } else {
-                    if (wv & 1) tile_axon_cell<T, false>(a, cell, t);
-                    else tile_soma_cell<T, false>(a, cell, t);
-                    if (wv & 1) tile_dend_pre_cell<T, false>(a, cell, t);
-                }
-            }
-        }
-        return;
-    }
-    const T *__restrict__ vin = a.vd[t & 1];
-    const TileDesc td = a.tiles[blockIdx.x];
     // the wavefront index is uniform: make it scalar so that the slice
     // pointers and widths below live in SGPRs and the row loads use a
     // scalar base with a 32-bit lane offset
@@ -1387,12 +1371,234 @@ __global__ __launch_bounds__(NT, (sizeof(T) == 4 && !HYB) ? (AWC ? 4 : 8) : 1) v
                     acc.merge(os, oc);
                 }
             }
-            if (l == 0) part_store<T, KAHAN>(a.part, (int64_t)blockIdx.x * a.rcells + rloc, acc);
+            if (l == 0) sink(rloc, acc);
         }
         sl = sn;
         rloc = rloc_n;
         d = d_n;
     }
+}
+
+/* One workgroup per tile (tile_rows), the partial pairs stored to
+ * part[tile][receiver]; workgroups past the tiles update somata, axons and
+ * dendrite gates.  The AWC variant's launch bounds leave room for the
+ * prefetched ids (four waves per SIMD rather than eight; it compiles to
+ * 60 VGPRs, still eight). */
+template <typename T, int L, bool KAHAN, int NT, bool HYB = false, bool AWC = false>
+__global__ __launch_bounds__(NT, (sizeof(T) == 4 && !HYB) ? (AWC ? 4 : 8) : 1) void k_tile_gj(StepArgs<T> a, int64_t k)
+{
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
+    T *vch = reinterpret_cast<T *>(dyn_lds);
+    const int64_t t = (a.d_step ? *a.d_step : 0) + k;
+    if ((int)blockIdx.x >= a.ntiles) {
+        // workgroups past the tiles: somata (even waves), axons and the
+        // dendrites up to their voltage (odd waves) of groups of NT/2 cells,
+        // which need no gap-junction current, so they run beside the tiles;
+        // workgroup g takes groups g, g + nsa, ...
+        const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+        for (int64_t g = (int64_t)blockIdx.x - a.ntiles; g * (NT / 2) < a.n; g += a.nsa) {
+            const int64_t cell = (g * (NT / 128) + (wv >> 1)) * 64 + (threadIdx.x & 63);
+            if (cell < a.n) {
+                // even waves: somata; odd waves: axons, then the dendrites up
+                // to their voltage (about 12 exp each way)
+                if (a.fast_math) {
+                    if (wv & 1) tile_axon_cell<T, true>(a, cell, t);
+                    else tile_soma_cell<T, true>(a, cell, t);
+                    if (wv & 1) tile_dend_pre_cell<T, true>(a, cell, t);
+                } else {
+                    if (wv & 1) tile_axon_cell<T, false>(a, cell, t);
+                    else tile_soma_cell<T, false>(a, cell, t);
+                    if (wv & 1) tile_dend_pre_cell<T, false>(a, cell, t);
+                }
+            }
+        }
+        return;
+    }
+    const int64_t ti = blockIdx.x;
+    tile_rows<T, L, KAHAN, NT, HYB, AWC>(a, a.tiles[ti], a.vd[t & 1], vch,
+                                         [&](int rloc, const Acc<T, KAHAN> &acc) {
+                                             part_store<T, KAHAN>(a.part, ti * a.rcells + rloc, acc);
+                                         });
+}
+
+/* The rows of one swept tile.  A swept tile's rows are short (a block's
+ * junctions spread over every chunk), so the time goes to load latency, not
+ * to summing: each wavefront runs its slices as a pipeline, loading the
+ * receiver's V_dend with the first kUnroll id / conductance slots DB = 1
+ * slice ahead, and its receiver, degree and slice bounds one slice before
+ * that (DB 2 took 1M cells of degree 50 from 309 to 330 us per step);
+ * longer rows load the rest in place (tile_row_terms from slot kUnroll).
+ * The terms are added in slot order with the checks of tile_row_terms, so
+ * each row's pair is the one tile_rows makes, and it is merged into the
+ * receiver's accumulator in LDS. */
+template <typename T, int L, bool KAHAN, int NT, bool HYB>
+__device__ __forceinline__ void sweep_tile_rows(const StepArgs<T> &a, const TileDesc td, const T *__restrict__ vin,
+                                                T *__restrict__ vch, typename PartOf<T, KAHAN>::type *accl)
+{
+    constexpr int CPW = 64 / L, NW = NT / 64;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int cw = lane / L, l = lane % L;
+    const int loff = cw * L + l;
+    const int64_t cellb = (int64_t)td.rb * a.rcells;
+    const int nsl = td.nsl;
+    const bool wct = td.wconst != 0;
+    const T wc = (T)td.wc;
+    const auto snake = [&](int r) { return r * NW + ((r & 1) ? NW - 1 - wave : wave); };
+    constexpr int DB = 1;
+    // stage A (DB + 1 slices ahead): receiver, degree, first entry, width
+    struct SA {
+        int rloc = 0, d = 0, W = 0;
+        int64_t b0 = 0;
+    };
+    // stage B (DB slices ahead): the receiver's V_dend, the first slots
+    struct SB {
+        T vi = 0;
+        int jv[kUnroll];
+        T wv[kUnroll];
+    };
+    const auto load_a = [&](int sl, SA &x) {
+        if (sl < nsl) {
+            const int64_t slot = td.rl0 + (int64_t)sl * CPW + cw;
+            x.rloc = a.rlist[slot];
+            x.d = a.rdeg[slot];
+            x.b0 = a.tsp[td.sl0 + sl];
+            x.W = (int)((a.tsp[td.sl0 + sl + 1] - x.b0) >> 6);
+        }
+    };
+    const auto load_b = [&](int sl, const SA &x, SB &y) {
+        if (sl < nsl) {
+            // receivers past the network's end (last block) have degree 0
+            // and read nothing
+            if (x.d > 0) y.vi = vin[cellb + x.rloc];
+            const uint16_t *ib = a.tcol + x.b0;
+            const T *wb = a.tw + x.b0;
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) {
+                if (u < x.W) {
+                    y.jv[u] = (int)ld_off(ib + u * 64, (uint32_t)loff * (uint32_t)sizeof(uint16_t));
+                    if (!wct) y.wv[u] = ld_off(wb + u * 64, (uint32_t)loff * (uint32_t)sizeof(T));
+                }
+            }
+        }
+    };
+    SA av[DB + 2];
+    SB bv[DB + 1];
+#pragma unroll
+    for (int i = 0; i <= DB; i++) load_a(snake(i), av[i]);
+#pragma unroll
+    for (int i = 0; i < DB; i++) load_b(snake(i), av[i], bv[i]);
+    tile_chunk_to_lds<T, NT>(vin + td.s0, vch, td.cn, tid);
+    __syncthreads();
+    for (int r = 0; r * NW < nsl; r++) {
+        const int sl = snake(r);
+        load_a(snake(r + DB + 1), av[DB + 1]);
+        load_b(snake(r + DB), av[DB], bv[DB]);
+        const SA &a0 = av[0];
+        const SB &b0 = bv[0];
+        if (sl < nsl) {
+            const int d = a0.d, W = a0.W;
+            Acc<T, KAHAN> acc;
+            int nvm = 0;
+            if constexpr (sizeof(T) == 4) {
+                int m = d > l ? (d - l + L - 1) / L : 0;
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) m = min(m, __shfl_xor(m, off));
+                nvm = __builtin_amdgcn_readfirstlane(m);
+            }
+            if (d > 0) {
+                const T vi = b0.vi;
+                const bool wide = HYB && a.deg[cellb + a0.rloc] >= a.fp64_deg;
+                double accd = 0.0;
+                T vv[kUnroll];
+#pragma unroll
+                for (int u = 0; u < kUnroll; u++)
+                    if (u < W) vv[u] = vch[b0.jv[u]];
+#pragma unroll
+                for (int u = 0; u < kUnroll; u++) {
+                    if (u < W) {
+                        const bool ok = u * L + l < d;
+                        if (HYB && wide) {
+                            const double term = gj_term((double)vi, (double)vv[u], (double)(wct ? wc : b0.wv[u]));
+                            accd += ok ? term : 0.0;
+                        } else {
+                            const T term = gj_term(vi, vv[u], wct ? wc : b0.wv[u]);
+                            acc.add(ok ? term : T(0));
+                        }
+                    }
+                }
+                if (W > kUnroll) {
+                    if (wct)
+                        tile_row_terms<T, KAHAN, L, true, HYB, 1>(acc, accd, wide, vi, d, l, W, nvm, kUnroll, loff,
+                                                                  a.tcol + a0.b0, a.tw + a0.b0, wc, vch);
+                    else
+                        tile_row_terms<T, KAHAN, L, false, HYB, 1>(acc, accd, wide, vi, d, l, W, nvm, kUnroll, loff,
+                                                                   a.tcol + a0.b0, a.tw + a0.b0, wc, vch);
+                }
+                if (HYB && wide) {
+                    acc.s = (T)accd;
+                    acc.c = (T)(accd - (double)acc.s);
+                }
+            }
+            if constexpr (L > 1) {
+#pragma unroll
+                for (int off = L / 2; off > 0; off >>= 1) {
+                    T os = shfl_xor(acc.s, off);
+                    T oc = KAHAN ? shfl_xor(acc.c, off) : T(0);
+                    acc.merge(os, oc);
+                }
+            }
+            if (l == 0) {
+                if constexpr (KAHAN) {
+                    const float2 v = accl[a0.rloc];
+                    Acc<T, KAHAN> m;
+                    m.s = v.x;
+                    m.c = v.y;
+                    m.merge(acc.s, acc.c);
+                    accl[a0.rloc] = make_float2(m.s, m.c);
+                } else {
+                    accl[a0.rloc] += acc.s;   // Acc::merge without compensation
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i <= DB; i++) av[i] = av[i + 1];
+#pragma unroll
+        for (int i = 0; i < DB; i++) bv[i] = bv[i + 1];
+    }
+}
+
+/* The tiled layout swept by one workgroup per receiver block: the workgroup
+ * walks the block's tiles in ascending chunk order, and each receiver's tile
+ * pair is merged at once into the receiver's accumulator in LDS, in the order
+ * tiled_ic merges stored partials, so the sums are the same bit for bit.  One
+ * pair per receiver is stored at the end (part[block][receiver]).  For
+ * networks whose blocks draw on many chunks: the per-tile partials would
+ * cross HBM twice, here they stay in LDS.  LDS: R accumulators, then the
+ * chunk. */
+template <typename T, int L, bool KAHAN, int NT, bool HYB = false>
+__global__ __launch_bounds__(NT, 1) void k_tile_sweep(StepArgs<T> a, int64_t k)
+{
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
+    using P = typename PartOf<T, KAHAN>::type;
+    P *accl = reinterpret_cast<P *>(dyn_lds);
+    T *vch = reinterpret_cast<T *>(dyn_lds + (size_t)a.rcells * sizeof(P));   // R is a multiple of 64
+    const int64_t t = (a.d_step ? *a.d_step : 0) + k;
+    const T *__restrict__ vin = a.vd[t & 1];
+    const int64_t rb = blockIdx.x;
+    for (int r = threadIdx.x; r < a.rcells; r += NT) {
+        if constexpr (KAHAN) accl[r] = make_float2(0.0f, 0.0f);
+        else accl[r] = T(0);
+    }
+    const int32_t t0 = a.rb_tile0[rb], t1 = a.rb_tile0[rb + 1];
+    for (int32_t ti = t0; ti < t1; ti++) {
+        // the last tile's rows are done with the chunk buffer (and, before
+        // the first tile, the accumulators are cleared)
+        __syncthreads();
+        sweep_tile_rows<T, L, KAHAN, NT, HYB>(a, a.tiles[ti], vin, vch, accl);
+    }
+    __syncthreads();
+    P *__restrict__ part = reinterpret_cast<P *>(a.part);
+    for (int r = threadIdx.x; r < a.rcells; r += NT) part[rb * a.rcells + r] = accl[r];
 }
 
 /* Every cell alone for steps[cell] steps: no gap-junction current, no

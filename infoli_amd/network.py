@@ -18,7 +18,7 @@ FP64_DEGREE_DEFAULT = 1280   # FP32_KAHAN: cells of at least this degree sum FP6
 _PREC_NAMES = {"fp64": FP64, "fp32": FP32, "fp32k": FP32_KAHAN, "fp32_kahan": FP32_KAHAN}
 
 EXEC_AUTO, EXEC_STEP, EXEC_GRAPH, EXEC_PERSISTENT = 0, 1, 2, 3
-GATHER_DIRECT, GATHER_LDS, GATHER_STAGED, GATHER_TILED = 0, 1, 2, 3
+GATHER_DIRECT, GATHER_LDS, GATHER_STAGED, GATHER_TILED, GATHER_SWEEP = 0, 1, 2, 3, 4
 
 # per-cell variables, bin/default.conf lines 24-42 order
 VARS = ("V_dend", "Calcium_r", "Potassium_s", "Hcurrent_q", "Ca2Plus", "I_CaH",

@@ -197,6 +197,110 @@ def test_constant_conductance_tiles(prec, L, p):
     assert np.abs(outs[0][0] - ref).max() < 2e-4
 
 
+def _isolated_blocks_case(n=6000, p=0.01, seed=41, lo=1024, hi=2048):
+    """Random symmetric graph whose cells [lo, hi) are isolated: with
+    512-cell receiver blocks, two blocks have no junctions at all."""
+    rp, col, w = I.gen_symmetric(seed, n, p, 0.02, 0.06)
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    keep = ~(((rows >= lo) & (rows < hi)) | ((col >= lo) & (col < hi)))
+    rp2 = np.zeros(n + 1, np.int64)
+    rp2[1:] = np.cumsum(np.bincount(rows[keep], minlength=n))
+    gcal = I.gen_uniform(seed, 2, n, 0.5, 1.1)
+    iapp = I.gen_uniform(seed, 3, n, -1.0, 3.0)
+    return rp2, col[keep].astype(np.int32), w[keep], gcal, iapp
+
+
+@pytest.mark.parametrize("prec,fm", [("fp64", 0), ("fp32", 0), ("fp32k", 0), ("fp32k", 1)])
+@pytest.mark.parametrize("L", [1, 4])
+def test_swept_tiles_equal_tiles(prec, fm, L):
+    """One workgroup per receiver block walking its chunks, each tile's
+    (sum, compensation) pair merged in LDS as it comes, gives the stored
+    per-tile partials' sums bit for bit: same tiles, same rows, same merge
+    order.  Two receiver blocks have no junctions, and their sums stay zero;
+    the trace matches the oracle."""
+    say(f"swept tiles {prec} fast_math {fm} L={L}")
+    n, steps = 6000, 150
+    rp, col, w, gcal, iapp = _isolated_blocks_case(n)
+    outs = []
+    for gather in (I.GATHER_TILED, I.GATHER_SWEEP):
+        net = make_net(n, prec, rp, col, w, gcal, iapp)
+        # 2 KiB chunks: 12 (FP32) or 24 (FP64) tiles per block
+        net.set_exec(mode=I.EXEC_GRAPH, gather=gather, lds_kb=2, tile_cells=512, block=1024,
+                     lanes_per_cell=L, ode_split=1, steps_per_launch=40, fast_math=fm)
+        assert net.exec["gather"] == gather
+        outs.append((net.advance(steps, trace_stride=3), np.stack([net.get_state(v) for v in I.VARS])))
+        net.close()
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
+    if not fm:
+        ref, _ = oracle_ref(prec, n, steps, rp, col, w, gcal, iapp, stride=3)
+        assert np.abs(outs[1][0] - ref).max() < (1e-10 if prec == "fp64" else 2e-4)
+
+
+def test_swept_tiles_fp64_degree_budget():
+    """FP32 compensated with a degree budget of 80: the swept tiles sum the
+    high-degree cells' terms in FP64 as the stored tiles do, bit for bit,
+    and the trace matches the oracle."""
+    say("swept tiles with the fp64 degree budget")
+    n, steps = 1200, 200
+    rp, col, w = _mixed_degree_case(n)
+    gcal = I.gen_uniform(7, 2, n, 0.5, 1.1)
+    iapp = I.gen_uniform(7, 3, n, -1.0, 3.0)
+    outs = []
+    for gather in (I.GATHER_TILED, I.GATHER_SWEEP):
+        net = make_net(n, "fp32k", rp, col, w, gcal, iapp)
+        net.set_fp64_degree(80)
+        net.set_exec(mode=I.EXEC_GRAPH, gather=gather, lds_kb=1, tile_cells=256, block=1024,
+                     lanes_per_cell=2, ode_split=0, steps_per_launch=50)
+        outs.append(net.advance(steps, trace_stride=10))
+        net.close()
+    assert np.array_equal(outs[0], outs[1])
+    ref, _ = oracle_ref("fp32k", n, steps, rp, col, w, gcal, iapp, fp64_degree=80, stride=10)
+    assert np.abs(outs[1] - ref).max() < 2e-4
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32k", 2e-4), ("fp32", 2e-4)])
+def test_swept_default_large_network_vs_oracle(prec, tol):
+    """500,000 cells of mean degree 50 (25M entries, 200 MB of FP32 lists,
+    beyond the Infinity Cache): the heuristic sweeps the tiles, one block
+    of at most n / CUs receivers per CU, and the trace matches the oracle.
+    In FP64 the direct gather stays the choice."""
+    say(f"swept default 500k {prec}")
+    n, steps = 500000, 20
+    rp, col, w = I.gen_symmetric(43, n, 50.0 / (n - 1), 0.02, 0.06)
+    gcal = I.gen_uniform(43, 2, n, 0.5, 1.1)
+    iapp = I.gen_normal(43, 3, n, 1.0, 1.0)
+    net = make_net(n, prec, rp, col, w, gcal, iapp)
+    e = net.exec
+    assert e["gather"] == I.GATHER_SWEEP and e["lanes_per_cell"] == 1 and e["block"] == 1024
+    assert e["tile_cells"] % 64 == 0 and e["tile_cells"] <= 4096
+    out = net.advance(steps, trace_stride=5)
+    net.close()
+    ref, _ = oracle_ref(prec, n, steps, rp, col, w, gcal, iapp, stride=5)
+    assert np.abs(out - ref).max() < tol
+    net = make_net(n, "fp64", rp, col, w, gcal, iapp)
+    assert net.exec["gather"] == I.GATHER_DIRECT
+    net.close()
+
+
+def test_swept_tiles_options_checked():
+    say("swept tiles option checks")
+    net = make_net(40000, "fp32k")
+    with pytest.raises(I.InfoliError):   # no somata and axons beside the swept tiles
+        net.set_exec(gather=I.GATHER_SWEEP, tile_cells=512, lds_kb=8, ode_split=2)
+    with pytest.raises(I.InfoliError):   # 16384 sums (128 KiB) and a 39 KiB chunk exceed the LDS
+        net.set_exec(gather=I.GATHER_SWEEP, tile_cells=16384, lds_kb=40, ode_split=1)
+    with pytest.raises(I.InfoliError):
+        net.set_exec(mode=I.EXEC_PERSISTENT, gather=I.GATHER_SWEEP, tile_cells=512, lds_kb=8, ode_split=1)
+    with pytest.raises(I.InfoliError):
+        net.set_exec(gather=I.GATHER_SWEEP + 1)
+    net.set_exec(mode=I.EXEC_GRAPH, gather=I.GATHER_SWEEP, tile_cells=16384, lds_kb=32, ode_split=1,
+                 block=256)
+    e = net.exec
+    assert e["gather"] == I.GATHER_SWEEP and e["block"] == 1024 and e["tile_cells"] == 16384
+    net.close()
+
+
 def test_compartment_split_options_checked():
     say("ode_split option checks")
     net = make_net(100, "fp32k")
