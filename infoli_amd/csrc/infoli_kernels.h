@@ -1422,15 +1422,17 @@ __global__ __launch_bounds__(NT, (sizeof(T) == 4 && !HYB) ? (AWC ? 4 : 8) : 1) v
 }
 
 /* The rows of one swept tile.  A swept tile's rows are short (a block's
- * junctions spread over every chunk), so the time goes to load latency, not
- * to summing: each wavefront runs its slices as a pipeline, loading the
- * receiver's V_dend with the first kUnroll id / conductance slots DB = 1
- * slice ahead, and its receiver, degree and slice bounds one slice before
- * that (DB 2 took 1M cells of degree 50 from 309 to 330 us per step);
- * longer rows load the rest in place (tile_row_terms from slot kUnroll).
- * The terms are added in slot order with the checks of tile_row_terms, so
- * each row's pair is the one tile_rows makes, and it is merged into the
- * receiver's accumulator in LDS. */
+ * junctions spread over every chunk: 1.6 per receiver per chunk at 1M
+ * cells of degree 50), so per-slice work and load latency dominate.  Each
+ * wavefront runs its slices as a pipeline, loading the receiver's V_dend
+ * with the first kUnroll id / conductance slots DB = 1 slice ahead, and its
+ * receiver, degree and slice bounds one slice before that; longer rows load
+ * the rest in place (tile_row_terms from slot kUnroll).  Measured at 1M
+ * cells of degree 50: 305 us per step; two slices ahead 330 us, four
+ * slices' loads at once before summing them 336 us, skipping the slices of
+ * width 0 311 us.  The terms are added in slot order with the checks of
+ * tile_row_terms, so each row's pair is the one tile_rows makes, and it is
+ * merged into the receiver's accumulator in LDS. */
 template <typename T, int L, bool KAHAN, int NT, bool HYB>
 __device__ __forceinline__ void sweep_tile_rows(const StepArgs<T> &a, const TileDesc td, const T *__restrict__ vin,
                                                 T *__restrict__ vch, typename PartOf<T, KAHAN>::type *accl)
