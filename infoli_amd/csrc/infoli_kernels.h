@@ -1425,12 +1425,13 @@ __global__ __launch_bounds__(NT, (sizeof(T) == 4 && !HYB) ? (AWC ? 4 : 8) : 1) v
  * junctions spread over every chunk: 1.6 per receiver per chunk at 1M
  * cells of degree 50), so per-slice work and load latency dominate.  Each
  * wavefront runs its slices as a pipeline, loading the receiver's V_dend
- * with the first kUnroll id / conductance slots DB = 1 slice ahead, and its
+ * with the first K = 4 id / conductance slots DB = 1 slice ahead, and its
  * receiver, degree and slice bounds one slice before that; longer rows load
- * the rest in place (tile_row_terms from slot kUnroll).  Measured at 1M
- * cells of degree 50: 305 us per step; two slices ahead 330 us, four
- * slices' loads at once before summing them 336 us, skipping the slices of
- * width 0 311 us.  The terms are added in slot order with the checks of
+ * the rest in place (tile_row_terms from slot K), and only they need the
+ * wavefront's fewest valid slots.  Measured at 1M cells of degree 50: 275 us
+ * per step; with 8 slots ahead and the minimum on every slice 305 us, two
+ * slices ahead 330 us, four slices' loads at once before summing them
+ * 336 us.  The terms are added in slot order with the checks of
  * tile_row_terms, so each row's pair is the one tile_rows makes, and it is
  * merged into the receiver's accumulator in LDS. */
 template <typename T, int L, bool KAHAN, int NT, bool HYB>
@@ -1446,7 +1447,7 @@ __device__ __forceinline__ void sweep_tile_rows(const StepArgs<T> &a, const Tile
     const bool wct = td.wconst != 0;
     const T wc = (T)td.wc;
     const auto snake = [&](int r) { return r * NW + ((r & 1) ? NW - 1 - wave : wave); };
-    constexpr int DB = 1;
+    constexpr int DB = 1, K = 4;   // K: slots loaded ahead
     // stage A (DB + 1 slices ahead): receiver, degree, first entry, width
     struct SA {
         int rloc = 0, d = 0, W = 0;
@@ -1455,8 +1456,8 @@ __device__ __forceinline__ void sweep_tile_rows(const StepArgs<T> &a, const Tile
     // stage B (DB slices ahead): the receiver's V_dend, the first slots
     struct SB {
         T vi = 0;
-        int jv[kUnroll];
-        T wv[kUnroll];
+        int jv[K];
+        T wv[K];
     };
     const auto load_a = [&](int sl, SA &x) {
         if (sl < nsl) {
@@ -1475,7 +1476,7 @@ __device__ __forceinline__ void sweep_tile_rows(const StepArgs<T> &a, const Tile
             const uint16_t *ib = a.tcol + x.b0;
             const T *wb = a.tw + x.b0;
 #pragma unroll
-            for (int u = 0; u < kUnroll; u++) {
+            for (int u = 0; u < K; u++) {
                 if (u < x.W) {
                     y.jv[u] = (int)ld_off(ib + u * 64, (uint32_t)loff * (uint32_t)sizeof(uint16_t));
                     if (!wct) y.wv[u] = ld_off(wb + u * 64, (uint32_t)loff * (uint32_t)sizeof(T));
@@ -1501,7 +1502,7 @@ __device__ __forceinline__ void sweep_tile_rows(const StepArgs<T> &a, const Tile
             const int d = a0.d, W = a0.W;
             Acc<T, KAHAN> acc;
             int nvm = 0;
-            if constexpr (sizeof(T) == 4) {
+            if (sizeof(T) == 4 && W > K) {
                 int m = d > l ? (d - l + L - 1) / L : 0;
 #pragma unroll
                 for (int off = 32; off > 0; off >>= 1) m = min(m, __shfl_xor(m, off));
@@ -1511,12 +1512,12 @@ __device__ __forceinline__ void sweep_tile_rows(const StepArgs<T> &a, const Tile
                 const T vi = b0.vi;
                 const bool wide = HYB && a.deg[cellb + a0.rloc] >= a.fp64_deg;
                 double accd = 0.0;
-                T vv[kUnroll];
+                T vv[K];
 #pragma unroll
-                for (int u = 0; u < kUnroll; u++)
+                for (int u = 0; u < K; u++)
                     if (u < W) vv[u] = vch[b0.jv[u]];
 #pragma unroll
-                for (int u = 0; u < kUnroll; u++) {
+                for (int u = 0; u < K; u++) {
                     if (u < W) {
                         const bool ok = u * L + l < d;
                         if (HYB && wide) {
@@ -1528,12 +1529,12 @@ __device__ __forceinline__ void sweep_tile_rows(const StepArgs<T> &a, const Tile
                         }
                     }
                 }
-                if (W > kUnroll) {
+                if (W > K) {
                     if (wct)
-                        tile_row_terms<T, KAHAN, L, true, HYB, 1>(acc, accd, wide, vi, d, l, W, nvm, kUnroll, loff,
+                        tile_row_terms<T, KAHAN, L, true, HYB, 1>(acc, accd, wide, vi, d, l, W, nvm, K, loff,
                                                                   a.tcol + a0.b0, a.tw + a0.b0, wc, vch);
                     else
-                        tile_row_terms<T, KAHAN, L, false, HYB, 1>(acc, accd, wide, vi, d, l, W, nvm, kUnroll, loff,
+                        tile_row_terms<T, KAHAN, L, false, HYB, 1>(acc, accd, wide, vi, d, l, W, nvm, K, loff,
                                                                    a.tcol + a0.b0, a.tw + a0.b0, wc, vch);
                 }
                 if (HYB && wide) {
