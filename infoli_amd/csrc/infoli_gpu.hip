@@ -1035,20 +1035,20 @@ void default_exec(infoli_ctx *c)
         c->exec.ode_split = (even_tiles ? spare : ntiles > (double)per_cu * c->num_cus) ? 2 : 1;
         return;
     }
-    // swept tiles for FP32 lists beyond the Infinity Cache with at least 45
+    // swept tiles for FP32 lists beyond the Infinity Cache with at least 36
     // junctions per cell: one 1024-thread workgroup per receiver block sums
     // the block's tiles in LDS.  A block's sums and chunk fill a CU's LDS, so
     // the blocks number at most the CUs (R = n / CUs, rounded up to 64), and
-    // the chunk takes the LDS the sums leave.  The time goes to per-tile
-    // latency, which grows with R, so R stays at most 4096.  Measured (FP32
-    // compensated, us per step, direct -> swept): 1M cells of degree 50
-    // 372 -> 305, of degree 100 700 -> 354, 500k of degree 50 184 -> 110;
-    // the direct gather stays ahead at degree 40 and below (1M cells:
-    // 307 -> 319 at 40, 177 -> 280 at 20), for 2M cells (1006 -> 1125 at
-    // R 7872) and in FP64 (573 -> 669 at 1M, degree 50)
-    if (is_f32(c) && stream_exceeds_cache(c) && mean_deg >= 45.0) {
+    // the chunk takes the LDS the sums leave, down to 96 KiB at R 8192.
+    // Measured (FP32 compensated, us per step, direct -> swept): 1M cells of
+    // degree 50 372 -> 275, of degree 100 681 -> 337, of degree 40
+    // 308 -> 274; 500k cells of degree 50 182 -> 114; 1.5M 599 -> 523; 2M
+    // 1008 -> 909.  The direct gather stays ahead at degree 30 (1M cells:
+    // 244 -> 265); in FP64 the two are within 3% (575 -> 561 at 1M cells of
+    // degree 50), and the direct gather is kept
+    if (is_f32(c) && stream_exceeds_cache(c) && mean_deg >= 36.0) {
         const int64_t R = round_up((c->n + c->num_cus - 1) / std::max(c->num_cus, 1), 64);
-        const int kb = R <= 4096 ? sweep_max_kb(c, (int)R) : 0;
+        const int kb = R <= 8192 ? sweep_max_kb(c, (int)R) : 0;
         if (kb >= 16) {
             c->exec.gather = GATHER_SWEEP;
             c->exec.lanes_per_cell = 1;
