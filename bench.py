@@ -27,6 +27,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# hardware exp2 (v_exp_f32) throughput measured on the MI355X by
+# `infoli_amd/bin/infoli_calib.x exp`: the ceiling of the gap-junction terms,
+# one exp each, when the sums run from on-chip memory (the dense network)
+VEXP_PEAK_PER_S = 13.0e12
 GATHER_NAMES = {0: "direct", 1: "lds", 2: "staged", 3: "tiled", 4: "sweep"}
 TILED_KERNELS = {0: "k_tile_gj+k_tile_ode", 1: "k_tile_gj+k_tile_ode3", 2: "k_tile_gj+k_tile_dend"}
 SWEEP_KERNELS = {0: "k_tile_sweep+k_tile_ode", 1: "k_tile_sweep+k_tile_ode3"}
@@ -260,9 +264,13 @@ def main():
     kname = kernel_name(exec_cfg)
     tr = measured_traffic(kname, wl["name"], args.precision)
     traffic = None if tr is None else tr * steps_per_launch
+    exp_rate = nnz / (mean_launch_ms * 1e-3)
     extra = {"device_ms": round(dev_ms, 3), "exec": exec_cfg,
              "mean_step_launches_ms": round(mean_launch_ms, 5),
-             "per_step_bracketed_ms": round(per_step_bracketed_ms, 5)}
+             "per_step_bracketed_ms": round(per_step_bracketed_ms, 5),
+             "exp_rate": {"achieved": exp_rate, "peak": VEXP_PEAK_PER_S, "unit": "exp/s",
+                          "frac": exp_rate / VEXP_PEAK_PER_S,
+                          "note": "one exp per neighbour entry per step; peak = measured v_exp_f32 rate"}}
     if not args.no_fp64 and args.precision != "fp64":
         print("# fp64 comparison run", file=sys.stderr, flush=True)
         n64 = make("fp64")
