@@ -674,6 +674,40 @@ def test_sparse_100k_full_size_precision():
     assert err <= 1e-3
 
 
+def test_large_1m_full_size_precision():
+    """The 1,000,000-cell sparse configuration (mean degree 50) at full size,
+    each precision under its default gather: FP64 (direct) equals the FP64
+    oracle after one step; over the first 100 ms (2000 steps) the FP32 runs
+    (swept tiles), compensated and plain, stay within 1e-3 mV of FP64."""
+    say("large_1m full size")
+    wl = W.large_1m()
+    n = wl["n"]
+    nets = {}
+    for prec in ("fp64", "fp32k", "fp32"):
+        net = I.Network(n, precision=prec)
+        W.load_into(net, wl)
+        nets[prec] = net
+    assert nets["fp64"].exec["gather"] == I.GATHER_DIRECT
+    assert nets["fp32k"].exec["gather"] == I.GATHER_SWEEP and nets["fp32"].exec["gather"] == I.GATHER_SWEEP
+    one64 = nets["fp64"].advance(1, trace_stride=1)
+    st = O.initial_state(n)
+    st[O.G_CAL] = wl["gcal"]
+    ref, _ = O.run(O.FP64, st, 1, wl["row_ptr"], wl["col"], wl["w"], iapp_base=wl["iapp_base"])
+    assert np.abs(one64 - ref).max() < 1e-12
+    nets["fp32k"].advance(1)
+    nets["fp32"].advance(1)
+    err = {"fp32k": 0.0, "fp32": 0.0}
+    for _ in range(4):   # 4 x ~500 steps, V_axon every 25 steps, to bound host memory
+        t64 = nets["fp64"].advance(500, trace_stride=25)
+        for prec in err:
+            t32 = nets[prec].advance(500, trace_stride=25)
+            err[prec] = max(err[prec], float(np.abs(t64 - t32).max()))
+    say(f"  max |dV| vs FP64 over 100 ms: compensated {err['fp32k']:.3e} mV, plain {err['fp32']:.3e} mV")
+    for net in nets.values():
+        net.close()
+    assert err["fp32k"] <= 1e-3 and err["fp32"] <= 1e-3
+
+
 def test_dense_10k_full_size_precision():
     """The 10,000-cell dense configuration (about 1,000 junctions per cell,
     every one of conductance 0.04) at full size: over the first 100 ms
