@@ -257,7 +257,8 @@ typedef struct infoli_exec {
                                 cell; 1 one wavefront per compartment; 2 somata,
                                 axons and dendrite gates beside the tiles, the
                                 dendrite voltages after (same results; not
-                                with the sweep)                              */
+                                with the sweep, where 0 updates each block's
+                                cells in the sweeping kernel itself)         */
     int     fast_math;       /* tiled, FP32: hardware exp2 and reciprocal in
                                 the compartment update (0 = libm expf and IEEE
                                 division, the reference's arithmetic)        */

@@ -255,6 +255,7 @@ StepArgs<T> make_args(const infoli_ctx *c, int64_t stride)
         a.ntiles = (int32_t)c->tl_ntiles;
         a.nsa = (int32_t)c->tl_nsa;
         a.fast_math = c->exec.fast_math ? 1 : 0;
+        a.sweep_ode = c->exec.gather == GATHER_SWEEP && c->exec.ode_split == 0 ? 1 : 0;
         a.fp64_deg = hyb_active(c) ? (int32_t)std::min<int64_t>(c->fp64_deg, INT32_MAX) : 0;
         a.rcells = c->tiled_R;
     }
@@ -425,11 +426,13 @@ int launch_step(infoli_ctx *c, const void *args, int64_t k)
         int64_t kk = k;
         void *kargs[] = {const_cast<void *>(args), &kk};
         if (c->exec.gather == GATHER_SWEEP) {
-            // one workgroup per receiver block, then the compartments
+            // one workgroup per receiver block; the compartments in the same
+            // kernel (ode_split 0) or after it (1)
             const void *fs = sweep_kernel(c);
             const size_t lds = sweep_lds_bytes(c, c->tiled_R, c->tl_C);
             allow_dyn_lds(fs, lds);
             HIPCHK(hipLaunchKernel(fs, dim3((unsigned)c->tl_nrb), dim3(1024), kargs, lds, c->stream));
+            if (c->exec.ode_split == 0) return INFOLI_OK;   // the compartments ran in the sweep
         } else {
             const void *fg = tile_kernel(c);
             const size_t lds = (size_t)c->tl_C * c->tsize;
@@ -1045,7 +1048,9 @@ void default_exec(infoli_ctx *c)
     // 308 -> 274; 500k cells of degree 50 182 -> 114; 1.5M 599 -> 523; 2M
     // 1008 -> 909.  The direct gather stays ahead at degree 30 (1M cells:
     // 244 -> 265); in FP64 the two are within 3% (575 -> 561 at 1M cells of
-    // degree 50), and the direct gather is kept
+    // degree 50), and the direct gather is kept.  The compartments run in the
+    // sweeping kernel (ode_split 0): 276 -> 274 us at 1M cells, 105 -> 102.5
+    // at 500k, against k_tile_ode3 after it
     if (is_f32(c) && stream_exceeds_cache(c) && mean_deg >= 36.0) {
         const int64_t R = round_up((c->n + c->num_cus - 1) / std::max(c->num_cus, 1), 64);
         const int kb = R <= 8192 ? sweep_max_kb(c, (int)R) : 0;
@@ -1055,7 +1060,7 @@ void default_exec(infoli_ctx *c)
             c->exec.block = 1024;
             c->exec.tile_cells = (int)R;
             c->exec.lds_kb = kb;
-            c->exec.ode_split = 1;
+            c->exec.ode_split = 0;
             return;
         }
     }
@@ -1985,7 +1990,7 @@ extern "C" int infoli_autotune(infoli_t *c, int64_t trial_steps, infoli_exec *ch
                 if (R < 64 || R > 16384 || R % 64 || c->n < (int64_t)16 * R) continue;
                 const int kb = sweep_max_kb(c, R);
                 if (kb < 1) continue;
-                const infoli_exec e{INFOLI_EXEC_GRAPH, L, 1024, spl, GATHER_SWEEP, kb, R, 1, 0};
+                const infoli_exec e{INFOLI_EXEC_GRAPH, L, 1024, spl, GATHER_SWEEP, kb, R, 0, 0};
                 if (memcmp(&e, &h, sizeof e) != 0) cands.push_back(e);
             }
         }

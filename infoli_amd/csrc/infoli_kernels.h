@@ -115,6 +115,8 @@ struct StepArgs {
     T *dpre;                   // ode_split 2: [2][npad] per cell the dendrite's current
                                // sum before I_c, and I_h (written beside the tiles)
     int32_t rcells;            // receivers per block (multiple of 64)
+    int32_t sweep_ode;         // swept tiles: each workgroup updates its block's cells
+                               // after their sums (no partials, no compartment kernel)
     Model<T> m;
 };
 
@@ -1574,7 +1576,8 @@ __device__ __forceinline__ void sweep_tile_rows(const StepArgs<T> &a, const Tile
  * walks the block's tiles in ascending chunk order, and each receiver's tile
  * pair is merged at once into the receiver's accumulator in LDS, in the order
  * tiled_ic merges stored partials, so the sums are the same bit for bit.  One
- * pair per receiver is stored at the end (part[block][receiver]).  For
+ * pair per receiver is stored at the end (part[block][receiver]), or, with
+ * sweep_ode, the workgroup updates its block's cells itself.  For
  * networks whose blocks draw on many chunks: the per-tile partials would
  * cross HBM twice, here they stay in LDS.  LDS: R accumulators, then the
  * chunk. */
@@ -1600,6 +1603,27 @@ __global__ __launch_bounds__(NT, 1) void k_tile_sweep(StepArgs<T> a, int64_t k)
         sweep_tile_rows<T, L, KAHAN, NT, HYB>(a, a.tiles[ti], vin, vch, accl);
     }
     __syncthreads();
+    if (a.sweep_ode) {
+        // the block's cells, one thread each, with I_c = the merged pair's
+        // value (what tiled_ic returns for the stored pair)
+        for (int r = threadIdx.x; r < a.rcells; r += NT) {
+            const int64_t cell = rb * a.rcells + r;
+            if (cell >= a.n) break;
+            T ic;
+            if constexpr (KAHAN) {
+                const float2 v = accl[r];
+                ic = v.x + v.y;
+            } else {
+                ic = accl[r];
+            }
+            typename CellOf<T>::type cs;
+            load_cell(a, cell, t, cs);
+            if (a.fast_math) ode<true>(a.m, cs, ic, iapp_of(a, cell, t));
+            else ode<false>(a.m, cs, ic, iapp_of(a, cell, t));
+            store_cell(a, cell, t, cs);
+        }
+        return;
+    }
     P *__restrict__ part = reinterpret_cast<P *>(a.part);
     for (int r = threadIdx.x; r < a.rcells; r += NT) part[rb * a.rcells + r] = accl[r];
 }

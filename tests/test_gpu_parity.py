@@ -213,14 +213,17 @@ def _isolated_blocks_case(n=6000, p=0.01, seed=41, lo=1024, hi=2048):
 @pytest.mark.parametrize("prec,fm", [("fp64", 0), ("fp32", 0), ("fp32k", 0), ("fp32k", 1)])
 @pytest.mark.parametrize("L", [1, 4])
 @pytest.mark.parametrize("R,kb", [(512, 2), (8192, 4)])
-def test_swept_tiles_equal_tiles(prec, fm, L, R, kb):
+@pytest.mark.parametrize("split", [0, 1])
+def test_swept_tiles_equal_tiles(prec, fm, L, R, kb, split):
     """One workgroup per receiver block walking its chunks, each tile's
     (sum, compensation) pair merged in LDS as it comes, gives the stored
     per-tile partials' sums bit for bit: same tiles, same rows, same merge
     order.  With 512-cell blocks two receiver blocks have no junctions, and
     their sums stay zero; one 8192-cell block holds 64 KiB of sums beside
-    its chunk.  The trace matches the oracle."""
-    say(f"swept tiles {prec} fast_math {fm} L={L} R={R}")
+    its chunk.  ode_split 0 updates the cells inside the sweeping kernel
+    (against k_tile_ode after the stored tiles), 1 in k_tile_ode3 after
+    both.  The trace matches the oracle."""
+    say(f"swept tiles {prec} fast_math {fm} L={L} R={R} split {split}")
     n, steps = 6000, 150
     rp, col, w, gcal, iapp = _isolated_blocks_case(n)
     outs = []
@@ -228,8 +231,9 @@ def test_swept_tiles_equal_tiles(prec, fm, L, R, kb):
         net = make_net(n, prec, rp, col, w, gcal, iapp)
         # 2 KiB chunks: 12 (FP32) or 24 (FP64) tiles per block; 4 KiB: 6 or 12
         net.set_exec(mode=I.EXEC_GRAPH, gather=gather, lds_kb=kb, tile_cells=R, block=1024,
-                     lanes_per_cell=L, ode_split=1, steps_per_launch=40, fast_math=fm)
+                     lanes_per_cell=L, ode_split=split, steps_per_launch=40, fast_math=fm)
         assert net.exec["gather"] == gather
+        say(f"  gather {gather}")
         outs.append((net.advance(steps, trace_stride=3), np.stack([net.get_state(v) for v in I.VARS])))
         net.close()
     assert np.array_equal(outs[0][0], outs[1][0])
