@@ -799,11 +799,13 @@ def test_isolated_steps_rand_init_vs_oracle(prec, tol):
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32k"])
 @pytest.mark.parametrize("split_at", [150, 151])
-def test_checkpoint_resume_bitwise(prec, split_at, tmp_path):
+@pytest.mark.parametrize("swept", [False, True])
+def test_checkpoint_resume_bitwise(prec, split_at, swept, tmp_path):
     """Save after `split_at` steps, continue; a fresh network loading the
     checkpoint continues identically (trace and all 19 variables), whichever
-    parity the step count has."""
-    say(f"checkpoint {prec} at {split_at}")
+    parity the step count has, under the default gather and under swept
+    tiles with the compartments in the sweeping kernel."""
+    say(f"checkpoint {prec} at {split_at} swept {swept}")
     n = 500
     rp, col, w, gcal, iapp = _random_case(n, p=0.05, seed=21)
     mask = I.gen_mask(21, 4, n, 0.4)
@@ -812,6 +814,9 @@ def test_checkpoint_resume_bitwise(prec, split_at, tmp_path):
     def build():
         net = make_net(n, prec, rp, col, w, gcal, iapp)
         net.set_iapp_pulse(3.0, 100, 260, mask)
+        if swept:
+            net.set_exec(mode=I.EXEC_GRAPH, gather=I.GATHER_SWEEP, tile_cells=128, lds_kb=1, ode_split=0,
+                         lanes_per_cell=1, steps_per_launch=40)
         return net
 
     a = build()
