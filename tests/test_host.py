@@ -49,6 +49,19 @@ def test_library_exports_every_declared_function():
     assert names <= exported, names - exported
 
 
+def test_python_constants_match_header():
+    """The binding's precision, launch-mode and gather codes and the FP64
+    degree default are the values include/infoli_gpu.h defines."""
+    hdr = open(os.path.join(os.path.dirname(__file__), "..", "include", "infoli_gpu.h")).read()
+    defs = {m.group(1): int(m.group(2))
+            for m in re.finditer(r"#define INFOLI_([A-Z0-9_]+)\s+(-?\d+)", hdr)}
+    names = ["FP64", "FP32", "FP32_KAHAN", "FP64_DEGREE_DEFAULT",
+             "EXEC_AUTO", "EXEC_STEP", "EXEC_GRAPH", "EXEC_PERSISTENT",
+             "GATHER_DIRECT", "GATHER_LDS", "GATHER_STAGED", "GATHER_TILED", "GATHER_SWEEP"]
+    for name in names:
+        assert getattr(I, name) == defs[name], name
+
+
 def test_version_and_strerror():
     L = I.library()
     assert b"gfx950" in L.infoli_version()
