@@ -1430,10 +1430,11 @@ __global__ __launch_bounds__(NT, (sizeof(T) == 4 && !HYB) ? (AWC ? 4 : 8) : 1) v
  * with the first K = 4 id / conductance slots DB = 1 slice ahead, and its
  * receiver, degree and slice bounds one slice before that; longer rows load
  * the rest in place (tile_row_terms from slot K), and only they need the
- * wavefront's fewest valid slots.  Measured at 1M cells of degree 50: 275 us
- * per step; with 8 slots ahead and the minimum on every slice 305 us, two
- * slices ahead 330 us, four slices' loads at once before summing them
- * 336 us.  The terms are added in slot order with the checks of
+ * wavefront's fewest valid slots.  Measured at 1M cells of degree 50, with
+ * the compartments in a kernel after the sweep: 275 us per step; 6 slots
+ * ahead 277, 8 slots 285 (305 with the minimum on every slice), two slices
+ * ahead 297, four slices' loads at once before summing them 336 (8 slots).
+ * The terms are added in slot order with the checks of
  * tile_row_terms, so each row's pair is the one tile_rows makes, and it is
  * merged into the receiver's accumulator in LDS. */
 template <typename T, int L, bool KAHAN, int NT, bool HYB>
