@@ -259,7 +259,7 @@ typedef struct infoli_exec {
                                 dendrite voltages after (same results; not
                                 with the sweep, where 0 updates each block's
                                 cells in the sweeping kernel itself)         */
-    int     fast_math;       /* tiled, FP32: hardware exp2 and reciprocal in
+    int     fast_math;       /* tiled / sweep, FP32: hardware exp2 and reciprocal in
                                 the compartment update (0 = libm expf and IEEE
                                 division, the reference's arithmetic)        */
 } infoli_exec;
