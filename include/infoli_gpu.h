@@ -194,7 +194,7 @@ int infoli_isolated_steps(infoli_t *ctx, const int64_t *steps);
  * k, u the cell's two 64-bit draws.  Host only. */
 int infoli_gen_rand_init(uint64_t seed, int64_t n, double delta, double *g_cal, int64_t *steps);
 
-/* ---- the step ----------------------------------------------------------- *//* ---- the step ----------------------------------------------------------- */
+/* ---- the step ----------------------------------------------------------- */
 
 /* Advance n_steps timesteps.  After step t (0-based, counted from context
  * creation) with t % trace_stride == 0, V_axon of every cell is appended to
@@ -340,6 +340,24 @@ int infoli_read_iapp_file(const char *path, int64_t n, int64_t max_steps,
                           double *table, int64_t *n_steps);
 
 /* ---- diagnostics ------------------------------------------------------- */
+
+/* Host-only check of the receiver-block x sender-chunk layout that
+ * INFOLI_GATHER_TILED / INFOLI_GATHER_SWEEP upload for this network and shape
+ * (needs no device).  Builds the tables with the library's own builder and
+ * checks them against the network: every sender id inside its tile's staged
+ * chunk (padding slots too, since they are read), each row's junctions in
+ * row order and each junction in exactly one tile, sender ranges ascending
+ * per block, slice widths, receiver lists that are permutations.  Then one
+ * step's scratch traffic as the kernels index it: every partial-sum and
+ * dendrite-sum slot the compartment kernel reads is written by the launch
+ * before it, and with ode_split 2 the `nsa` soma/axon workgroups update every
+ * soma and axon exactly once.  INFOLI_OK, or INFOLI_ERR_STATE with the first
+ * violation in infoli_last_error().  No reference counterpart: it checks the
+ * layout that replaces cellCompParams (src/openmp/infoli.h:79-83). */
+int infoli_tiled_layout_check(int64_t n_cells, const int64_t *row_ptr, const int32_t *col,
+                              const double *w, int precision, int gather,
+                              int lanes_per_cell, int tile_cells, int lds_kb,
+                              int ode_split, int block, int64_t nsa);
 
 const char *infoli_last_error(void);
 const char *infoli_strerror(int code);

@@ -35,6 +35,7 @@ from .network import (  # noqa: F401
     read_conn_triplets,
     read_iapp_file,
     read_values,
+    tiled_layout_check,
 )
 from . import workloads  # noqa: F401
 

@@ -363,9 +363,11 @@ bool is_f32(const infoli_ctx *c) { return c->prec != INFOLI_FP64; }
 size_t part_bytes(const infoli_ctx *c) { return c->prec == INFOLI_FP32_KAHAN ? 8 : c->tsize; }
 
 /* the sweep's LDS: one accumulator per receiver of the block, then the chunk */
+size_t sweep_lds_bytes_of(int R, int64_t C, size_t tsize, size_t psz) { return (size_t)R * psz + (size_t)C * tsize; }
+
 size_t sweep_lds_bytes(const infoli_ctx *c, int R, int64_t C)
 {
-    return (size_t)R * part_bytes(c) + (size_t)C * c->tsize;
+    return sweep_lds_bytes_of(R, C, c->tsize, part_bytes(c));
 }
 
 /* the largest chunk budget (KiB) that fits beside R receivers' sums */
@@ -491,6 +493,39 @@ int launch_persist(infoli_ctx *c, const void *args, int nsteps, int grid)
 __global__ void k_set_step(int64_t *d_step, int64_t v) { *d_step = v; }
 
 /* ------------------------------------------------------------------------ */
+/* copies and fills, all on the context's stream                             */
+/* ------------------------------------------------------------------------ */
+
+/* Every copy and fill of a context is issued on its stream and waited for
+ * there.  The stream is created non-blocking, so it is not ordered after the
+ * null stream: a hipMemset or hipMemcpy on the null stream may still be in
+ * flight (hipMemset of device memory returns before the fill completes) when
+ * the stream's next kernel or upload runs. */
+int h2d(infoli_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0) return INFOLI_OK;
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return INFOLI_OK;
+}
+
+int d2h(infoli_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0) return INFOLI_OK;
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return INFOLI_OK;
+}
+
+int zero_dev(infoli_ctx *c, void *dst, size_t bytes)
+{
+    if (bytes == 0) return INFOLI_OK;
+    HIPCHK(hipMemsetAsync(dst, 0, bytes, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return INFOLI_OK;
+}
+
+/* ------------------------------------------------------------------------ */
 /* sliced-ELL layout                                                         */
 /* ------------------------------------------------------------------------ */
 
@@ -525,10 +560,10 @@ int upload_sell(infoli_ctx *c, int L)
     HIPCHK(hipMalloc(&c->d_col, sizeof(int32_t) * col.size()));
     HIPCHK(hipMalloc(&c->d_w, sizeof(T) * w.size()));
     HIPCHK(hipMalloc(&c->d_slice_ptr, sizeof(int64_t) * sp.size()));
-    HIPCHK(hipMemcpy(c->d_col, col.data(), sizeof(int32_t) * col.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d_w, w.data(), sizeof(T) * w.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d_slice_ptr, sp.data(), sizeof(int64_t) * sp.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d_deg, deg.data(), sizeof(int32_t) * npad, hipMemcpyHostToDevice));
+    RCHK(h2d(c, c->d_col, col.data(), sizeof(int32_t) * col.size()));
+    RCHK(h2d(c, c->d_w, w.data(), sizeof(T) * w.size()));
+    RCHK(h2d(c, c->d_slice_ptr, sp.data(), sizeof(int64_t) * sp.size()));
+    RCHK(h2d(c, c->d_deg, deg.data(), sizeof(int32_t) * npad));
     c->sell_L = L;
     c->sell_entries = ne;
     return INFOLI_OK;
@@ -578,30 +613,58 @@ void free_tiled(infoli_ctx *c)
 }
 
 /* chunk of senders per tile for an LDS budget: a multiple of 64 cells */
-int64_t tiled_chunk(const infoli_ctx *c, int kb)
+int64_t chunk_cells(int64_t n, size_t tsize, int kb)
 {
-    const int64_t cap = (int64_t)kb * 1024 / (int64_t)c->tsize / 64 * 64;
-    const int64_t nch = (c->n + cap - 1) / cap;
-    return std::min<int64_t>(cap, round_up((c->n + nch - 1) / nch, 64));
+    const int64_t cap = (int64_t)kb * 1024 / (int64_t)tsize / 64 * 64;
+    const int64_t nch = (n + cap - 1) / cap;
+    return std::min<int64_t>(cap, round_up((n + nch - 1) / nch, 64));
 }
+
+int64_t tiled_chunk(const infoli_ctx *c, int kb) { return chunk_cells(c->n, c->tsize, kb); }
+
+/* receiver-major CSR of a network, as the library keeps it */
+struct CsrView {
+    int64_t n;
+    const int64_t *row_ptr;
+    const int32_t *col;
+    const double *w;
+};
+
+/* the host tables of the 2D-tiled layout (uploaded as they are) */
+template <typename T>
+struct TiledTables {
+    int64_t C = 0, nrb = 0;
+    std::vector<TileDesc> tiles;
+    std::vector<int32_t> rbt0;
+    std::vector<int64_t> tsp;
+    std::vector<uint16_t> rlist, rdeg, tcol;
+    std::vector<T> tw;
+};
 
 /* 2D-tiled layout: receiver blocks of R cells x sender chunks of C cells.
  * A tile exists for every (block, chunk) pair with a junction.  In a tile
  * the block's R receivers are listed by in-chunk degree, descending, and
  * their in-chunk junctions (in row order) are stored sliced-ELL with
- * 16-bit chunk-local sender ids. */
+ * 16-bit chunk-local sender ids.  Host only. */
 template <typename T>
-int upload_tiled(infoli_ctx *c, int L, int R, int kb)
+int build_tiled(const CsrView &g, size_t tsize, int L, int R, int kb, TiledTables<T> &tb)
 {
     const int CPW = 64 / L;
-    const int64_t n = c->n;
-    const int64_t C = tiled_chunk(c, kb);
+    const int64_t n = g.n;
+    const int64_t C = chunk_cells(n, tsize, kb);
     const int64_t nrb = (n + R - 1) / R;
-    std::vector<TileDesc> tiles;
-    std::vector<int32_t> rbt0(nrb + 1, 0);
-    std::vector<int64_t> tsp(1, 0);
-    std::vector<uint16_t> rlist, rdeg, tcol;
-    std::vector<T> tw;
+    std::vector<TileDesc> &tiles = tb.tiles;
+    std::vector<int32_t> &rbt0 = tb.rbt0;
+    std::vector<int64_t> &tsp = tb.tsp;
+    std::vector<uint16_t> &rlist = tb.rlist, &rdeg = tb.rdeg, &tcol = tb.tcol;
+    std::vector<T> &tw = tb.tw;
+    tiles.clear();
+    rbt0.assign(nrb + 1, 0);
+    tsp.assign(1, 0);
+    rlist.clear();
+    rdeg.clear();
+    tcol.clear();
+    tw.clear();
     for (int64_t rb = 0; rb < nrb; rb++) {
         rbt0[rb] = (int32_t)tiles.size();
         const int64_t c0 = rb * R, c1 = std::min<int64_t>(n, c0 + R);
@@ -609,9 +672,9 @@ int upload_tiled(infoli_ctx *c, int L, int R, int kb)
         // batches), one tile stages exactly that range; else the sender
         // chunks [k C, (k + 1) C)
         int64_t lo = n, hi = -1;
-        for (int64_t e = c->row_ptr[c0]; e < c->row_ptr[c1]; e++) {
-            lo = std::min<int64_t>(lo, c->col[e]);
-            hi = std::max<int64_t>(hi, c->col[e]);
+        for (int64_t e = g.row_ptr[c0]; e < g.row_ptr[c1]; e++) {
+            lo = std::min<int64_t>(lo, g.col[e]);
+            hi = std::max<int64_t>(hi, g.col[e]);
         }
         if (hi >= lo) lo = lo / 64 * 64;   // the LDS copy loads 16-byte vectors from the range start
         const bool compact = hi >= lo && hi - lo + 1 <= C;
@@ -623,9 +686,9 @@ int upload_tiled(infoli_ctx *c, int L, int R, int kb)
         std::vector<std::vector<int64_t>> ent(nchunks);
         std::vector<std::vector<int32_t>> entr(nchunks);
         for (int64_t i = c0; i < c1; i++)
-            for (int64_t e = c->row_ptr[i]; e < c->row_ptr[i + 1]; e++) {
-                ent[chunk_of(c->col[e])].push_back(e);
-                entr[chunk_of(c->col[e])].push_back((int32_t)(i - c0));
+            for (int64_t e = g.row_ptr[i]; e < g.row_ptr[i + 1]; e++) {
+                ent[chunk_of(g.col[e])].push_back(e);
+                entr[chunk_of(g.col[e])].push_back((int32_t)(i - c0));
             }
         for (int64_t ch = 0; ch < nchunks; ch++) {
             if (ent[ch].empty()) continue;
@@ -647,10 +710,10 @@ int upload_tiled(infoli_ctx *c, int L, int R, int kb)
             td.cn = (int32_t)chunk_cn(ch);
             td.pad = 0;
             // constant conductance over the tile (in the precision of the step)
-            const T w0 = (T)c->w[ent[ch][0]];
+            const T w0 = (T)g.w[ent[ch][0]];
             bool cw_all = true;
             for (int64_t e : ent[ch])
-                if ((T)c->w[e] != w0) { cw_all = false; break; }
+                if ((T)g.w[e] != w0) { cw_all = false; break; }
             td.wconst = cw_all ? 1 : 0;
             td.wc = (double)w0;
             td.sl0 = (int64_t)tsp.size() - 1;
@@ -675,8 +738,8 @@ int upload_tiled(infoli_ctx *c, int L, int R, int kb)
                     for (int p = 0; p < deg[r]; p++) {
                         const int64_t e = ent[ch][f + p];
                         const int64_t slot = base + (int64_t)(p / L) * 64 + cw * L + (p % L);
-                        tcol[slot] = (uint16_t)(c->col[e] - td.s0);
-                        tw[slot] = (T)c->w[e];
+                        tcol[slot] = (uint16_t)(g.col[e] - td.s0);
+                        tw[slot] = (T)g.w[e];
                     }
                 }
             }
@@ -689,37 +752,55 @@ int upload_tiled(infoli_ctx *c, int L, int R, int kb)
         tiles.push_back(td);
         std::fill(rbt0.begin(), rbt0.end(), 0);
     }
+    tb.C = C;
+    tb.nrb = nrb;
+    return INFOLI_OK;
+}
+
+/* partial-sum pairs the layout needs: one per receiver per tile, or one per
+ * receiver per block when swept */
+size_t part_slots(size_t ntiles, int64_t nrb, int R) { return std::max<size_t>(ntiles, (size_t)nrb) * (size_t)R; }
+
+template <typename T>
+int upload_tiled(infoli_ctx *c, int L, int R, int kb)
+{
+    TiledTables<T> tb;
+    RCHK(build_tiled<T>(CsrView{c->n, c->row_ptr.data(), c->col.data(), c->w.data()}, c->tsize, L, R, kb, tb));
+    const std::vector<TileDesc> &tiles = tb.tiles;
+    const int64_t nrb = tb.nrb;
     free_tiled(c);
     const size_t psz = part_bytes(c);
     std::vector<int32_t> rb_one(nrb + 1);
     for (int64_t rb = 0; rb <= nrb; rb++) rb_one[rb] = (int32_t)rb;
     HIPCHK(hipMalloc(&c->d_tiles, sizeof(TileDesc) * tiles.size()));
-    HIPCHK(hipMalloc(&c->d_rb_tile0, sizeof(int32_t) * rbt0.size()));
+    HIPCHK(hipMalloc(&c->d_rb_tile0, sizeof(int32_t) * tb.rbt0.size()));
     HIPCHK(hipMalloc(&c->d_rb_one, sizeof(int32_t) * rb_one.size()));
-    HIPCHK(hipMalloc(&c->d_tsp, sizeof(int64_t) * tsp.size()));
-    HIPCHK(hipMalloc(&c->d_rlist, sizeof(uint16_t) * std::max<size_t>(rlist.size(), 1)));
-    HIPCHK(hipMalloc(&c->d_rdeg, sizeof(uint16_t) * std::max<size_t>(rdeg.size(), 1)));
-    HIPCHK(hipMalloc(&c->d_tcol, sizeof(uint16_t) * std::max<size_t>(tcol.size(), 1)));
-    HIPCHK(hipMalloc(&c->d_tw, sizeof(T) * std::max<size_t>(tw.size(), 1)));
+    HIPCHK(hipMalloc(&c->d_tsp, sizeof(int64_t) * tb.tsp.size()));
+    HIPCHK(hipMalloc(&c->d_rlist, sizeof(uint16_t) * std::max<size_t>(tb.rlist.size(), 1)));
+    HIPCHK(hipMalloc(&c->d_rdeg, sizeof(uint16_t) * std::max<size_t>(tb.rdeg.size(), 1)));
+    HIPCHK(hipMalloc(&c->d_tcol, sizeof(uint16_t) * std::max<size_t>(tb.tcol.size(), 1)));
+    HIPCHK(hipMalloc(&c->d_tw, sizeof(T) * std::max<size_t>(tb.tw.size(), 1)));
     // partials: one per tile, or one per block when swept
-    HIPCHK(hipMalloc(&c->d_part, psz * std::max<size_t>(tiles.size(), (size_t)nrb) * (size_t)R));
+    const size_t pbytes = psz * part_slots(tiles.size(), nrb, R);
+    HIPCHK(hipMalloc(&c->d_part, pbytes));
     HIPCHK(hipMalloc(&c->d_dpre, 2 * (size_t)c->npad * c->tsize));
-    HIPCHK(hipMemcpy(c->d_tiles, tiles.data(), sizeof(TileDesc) * tiles.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d_rb_tile0, rbt0.data(), sizeof(int32_t) * rbt0.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d_rb_one, rb_one.data(), sizeof(int32_t) * rb_one.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d_tsp, tsp.data(), sizeof(int64_t) * tsp.size(), hipMemcpyHostToDevice));
-    if (!rlist.empty()) {
-        HIPCHK(hipMemcpy(c->d_rlist, rlist.data(), sizeof(uint16_t) * rlist.size(), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(c->d_rdeg, rdeg.data(), sizeof(uint16_t) * rdeg.size(), hipMemcpyHostToDevice));
-    }
-    if (!tcol.empty()) {
-        HIPCHK(hipMemcpy(c->d_tcol, tcol.data(), sizeof(uint16_t) * tcol.size(), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(c->d_tw, tw.data(), sizeof(T) * tw.size(), hipMemcpyHostToDevice));
-    }
+    // every slot a step reads is written earlier in that step (the layout
+    // check infoli_tiled_layout_check verifies it); zeroed all the same, so
+    // the scratch never carries a freed buffer's data
+    RCHK(zero_dev(c, c->d_part, pbytes));
+    RCHK(zero_dev(c, c->d_dpre, 2 * (size_t)c->npad * c->tsize));
+    RCHK(h2d(c, c->d_tiles, tiles.data(), sizeof(TileDesc) * tiles.size()));
+    RCHK(h2d(c, c->d_rb_tile0, tb.rbt0.data(), sizeof(int32_t) * tb.rbt0.size()));
+    RCHK(h2d(c, c->d_rb_one, rb_one.data(), sizeof(int32_t) * rb_one.size()));
+    RCHK(h2d(c, c->d_tsp, tb.tsp.data(), sizeof(int64_t) * tb.tsp.size()));
+    RCHK(h2d(c, c->d_rlist, tb.rlist.data(), sizeof(uint16_t) * tb.rlist.size()));
+    RCHK(h2d(c, c->d_rdeg, tb.rdeg.data(), sizeof(uint16_t) * tb.rdeg.size()));
+    RCHK(h2d(c, c->d_tcol, tb.tcol.data(), sizeof(uint16_t) * tb.tcol.size()));
+    RCHK(h2d(c, c->d_tw, tb.tw.data(), sizeof(T) * tb.tw.size()));
     c->tl_ntiles = (int64_t)tiles.size();
     c->tl_allwc = std::all_of(tiles.begin(), tiles.end(), [](const TileDesc &t) { return t.wconst != 0; });
     c->tl_nrb = nrb;
-    c->tl_C = C;
+    c->tl_C = tb.C;
     c->tiled_L = L;
     c->tiled_kb = kb;
     c->tiled_R = R;
@@ -782,19 +863,19 @@ int upload_staged(infoli_ctx *c, int kb)
     HIPCHK(hipMalloc(&c->d_blk_ncell, sizeof(int32_t) * nb));
     HIPCHK(hipMalloc(&c->d_blk_gbase, sizeof(int64_t) * (nb + 1)));
     HIPCHK(hipMalloc(&c->d_lrs, sizeof(int32_t) * npad));
-    HIPCHK(hipMemcpy(c->d_blk_cell0, b0.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d_blk_ncell, bn.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d_blk_gbase, gbase.data(), sizeof(int64_t) * (nb + 1), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d_lrs, lrs.data(), sizeof(int32_t) * npad, hipMemcpyHostToDevice));
+    RCHK(h2d(c, c->d_blk_cell0, b0.data(), sizeof(int32_t) * nb));
+    RCHK(h2d(c, c->d_blk_ncell, bn.data(), sizeof(int32_t) * nb));
+    RCHK(h2d(c, c->d_blk_gbase, gbase.data(), sizeof(int64_t) * (nb + 1)));
+    RCHK(h2d(c, c->d_lrs, lrs.data(), sizeof(int32_t) * npad));
     if (packed) {
         std::vector<uint32_t> l32(list.size());
         for (size_t k = 0; k < list.size(); k++)
             l32[k] = ((uint32_t)(list[k] >> 32) << sb) | (uint32_t)(list[k] & 0xFFFFFFFFu);
         HIPCHK(hipMalloc(&c->d_gl32, sizeof(uint32_t) * l32.size()));
-        HIPCHK(hipMemcpy(c->d_gl32, l32.data(), sizeof(uint32_t) * l32.size(), hipMemcpyHostToDevice));
+        RCHK(h2d(c, c->d_gl32, l32.data(), sizeof(uint32_t) * l32.size()));
     } else {
         HIPCHK(hipMalloc(&c->d_gl64, sizeof(uint64_t) * list.size()));
-        HIPCHK(hipMemcpy(c->d_gl64, list.data(), sizeof(uint64_t) * list.size(), hipMemcpyHostToDevice));
+        RCHK(h2d(c, c->d_gl64, list.data(), sizeof(uint64_t) * list.size()));
     }
     c->st_nblocks = nb;
     c->st_elds = elds;
@@ -1184,11 +1265,16 @@ extern "C" int infoli_create(infoli_t **out, const infoli_params *params, int64_
         hipMalloc(&c->d_flag, sizeof(int)) != hipSuccess)
         return fail(infoli_fail(INFOLI_ERR_NOMEM, "infoli_create: device allocation of %lld cells failed",
                                 (long long)n_cells));
-    if (hipMemset(c->d_state, 0, plane * kNV) != hipSuccess ||
-        hipMemset(c->d_vd[0], 0, plane) != hipSuccess || hipMemset(c->d_vd[1], 0, plane) != hipSuccess ||
-        hipMemset(c->d_vs[0], 0, plane) != hipSuccess || hipMemset(c->d_vs[1], 0, plane) != hipSuccess ||
-        hipMemset(c->d_va[0], 0, plane) != hipSuccess || hipMemset(c->d_va[1], 0, plane) != hipSuccess)
-        return fail(infoli_fail(INFOLI_ERR_HIP, "infoli_create: memset failed"));
+    // zeroed on the context's stream, ahead of the initial-state uploads on
+    // that stream (a null-stream hipMemset is not ordered before them, and a
+    // voltage plane zeroed after its upload drives the network to NaN)
+    for (void *p : {c->d_vd[0], c->d_vd[1], c->d_vs[0], c->d_vs[1], c->d_va[0], c->d_va[1]}) {
+        rc = zero_dev(c, p, plane);
+        if (rc) return fail(rc);
+    }
+    rc = zero_dev(c, c->d_state, plane * kNV);
+    if (rc == INFOLI_OK) rc = zero_dev(c, c->d_deg, sizeof(int32_t) * c->npad);
+    if (rc) return fail(rc);
 
     // initial state (OMP:267-292): float readings for FP32, double for FP64
     std::vector<double> vals(n_cells);
@@ -1341,6 +1427,7 @@ extern "C" int infoli_set_iapp(infoli_t *c, const double *base)
     infoli_clear_error();
     RCHK(validate_ctx(c, "infoli_set_iapp"));
     HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
     if (!base) {
         if (c->d_iapp_base) HIPCHK(hipFree(c->d_iapp_base));
         c->d_iapp_base = nullptr;
@@ -1360,9 +1447,10 @@ extern "C" int infoli_set_iapp_pulse(infoli_t *c, const uint8_t *mask, double am
     RCHK(validate_ctx(c, "infoli_set_iapp_pulse"));
     if (!std::isfinite(amp)) return infoli_fail(INFOLI_ERR_ARG, "pulse amplitude not finite");
     HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
     if (mask) {
         if (!c->d_mask) HIPCHK(hipMalloc(&c->d_mask, (size_t)c->n));
-        HIPCHK(hipMemcpy(c->d_mask, mask, (size_t)c->n, hipMemcpyHostToDevice));
+        RCHK(h2d(c, c->d_mask, mask, (size_t)c->n));
     } else if (c->d_mask) {
         HIPCHK(hipFree(c->d_mask));
         c->d_mask = nullptr;
@@ -1379,6 +1467,7 @@ extern "C" int infoli_set_iapp_table(infoli_t *c, int64_t t_first, int64_t n_ste
     infoli_clear_error();
     RCHK(validate_ctx(c, "infoli_set_iapp_table"));
     HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
     if (c->d_tab) HIPCHK(hipFree(c->d_tab));
     c->d_tab = nullptr;
     c->tab_n = 0;
@@ -1520,9 +1609,8 @@ extern "C" int infoli_isolated_steps(infoli_t *c, const int64_t *steps)
     int64_t *d_steps = nullptr;
     HIPCHK(hipMalloc(&d_steps, sizeof(int64_t) * c->n));
     int rc = INFOLI_OK;
-    if (hipMemcpy(d_steps, steps, sizeof(int64_t) * c->n, hipMemcpyHostToDevice) != hipSuccess) {
-        rc = infoli_fail(INFOLI_ERR_HIP, "infoli_isolated_steps: copy failed");
-    } else {
+    rc = h2d(c, d_steps, steps, sizeof(int64_t) * c->n);
+    if (rc == INFOLI_OK) {
         const dim3 grid((unsigned)((c->n + kBlock - 1) / kBlock));
         if (is_f32(c))
             hipLaunchKernelGGL(k_isolated<float>, grid, dim3(kBlock), 0, c->stream, make_args<float>(c, 0), c->step,
@@ -1942,7 +2030,7 @@ extern "C" int infoli_autotune(infoli_t *c, int64_t trial_steps, infoli_exec *ch
     std::vector<unsigned char> saved(plane * kNV);
     HIPCHK(hipStreamSynchronize(c->stream));
     for (int v = 0; v < kNV; v++)
-        HIPCHK(hipMemcpy(saved.data() + plane * v, var_ptr(c, v), plane, hipMemcpyDeviceToHost));
+        RCHK(d2h(c, saved.data() + plane * v, var_ptr(c, v), plane));
     const int64_t step0 = c->step;
     const infoli_exec orig = c->exec;
     const bool prof = c->profiling;
@@ -2034,11 +2122,188 @@ extern "C" int infoli_autotune(infoli_t *c, int64_t trial_steps, infoli_exec *ch
     drop_graph(c);
     c->step = step0;
     for (int v = 0; v < kNV; v++)
-        HIPCHK(hipMemcpy(var_ptr(c, v), saved.data() + plane * v, plane, hipMemcpyHostToDevice));
+        RCHK(h2d(c, var_ptr(c, v), saved.data() + plane * v, plane));
     c->profiling = prof;
     RCHK(ensure_layout(c));
     if (rc != INFOLI_OK) return infoli_fail(rc, "infoli_autotune: a trial failed (%s)", infoli_strerror(rc));
     infoli_clear_error();   // the messages of candidates that did not apply
     if (chosen) *chosen = best_e;
     return INFOLI_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* host check of the tiled layout                                            */
+/* ------------------------------------------------------------------------ */
+
+namespace {
+
+/* The tables of build_tiled against the network, and one step's scratch
+ * traffic as the kernels index it: which partial-sum and dendrite slots the
+ * first launch writes (k_tile_gj with its soma/axon workgroups, or
+ * k_tile_sweep) and which the second reads (k_tile_ode, k_tile_ode3 or
+ * k_tile_dend). */
+template <typename T>
+int check_tiled(const CsrView &g, size_t tsize, size_t psz, int gather, int L, int R, int kb, int split, int block,
+                int64_t nsa)
+{
+    TiledTables<T> tb;
+    RCHK(build_tiled<T>(g, tsize, L, R, kb, tb));
+    const int64_t n = g.n, npad = round_up(n, kBlock), nnz = g.row_ptr[n];
+    const int CPW = 64 / L;
+    const int64_t ntiles = (int64_t)tb.tiles.size(), nrb = tb.nrb, C = tb.C;
+    const bool any = nnz > 0;
+    if ((int64_t)tb.rbt0.size() != nrb + 1 || tb.rbt0[nrb] != (any ? ntiles : 0))
+        return infoli_fail(INFOLI_ERR_STATE, "layout: block table does not end at the tile count");
+    if (gather == GATHER_SWEEP && sweep_lds_bytes_of(R, C, tsize, psz) > kLdsBytes)
+        return infoli_fail(INFOLI_ERR_STATE, "layout: swept sums and chunk exceed the LDS");
+    int64_t covered = 0;
+    for (int64_t rb = 0; rb < nrb; rb++) {
+        int64_t prev_end = -1;
+        for (int32_t ti = tb.rbt0[rb]; ti < tb.rbt0[rb + 1]; ti++) {
+            const TileDesc &td = tb.tiles[ti];
+            if (td.rb != rb) return infoli_fail(INFOLI_ERR_STATE, "layout: tile %d lists block %d, not %lld", ti, td.rb, (long long)rb);
+            if (td.nsl * CPW != R) return infoli_fail(INFOLI_ERR_STATE, "layout: tile %d has %d slices", ti, td.nsl);
+            if (td.s0 < 0 || td.s0 % 64 || td.cn < 1 || td.s0 + td.cn > n || td.cn > C)
+                return infoli_fail(INFOLI_ERR_STATE, "layout: tile %d stages senders [%lld, %lld) of %lld (chunk %lld)", ti,
+                                   (long long)td.s0, (long long)(td.s0 + td.cn), (long long)n, (long long)C);
+            // sender ranges ascend within the block: partials merge in ascending sender order
+            if (td.s0 < prev_end) return infoli_fail(INFOLI_ERR_STATE, "layout: tile %d's senders overlap or precede the last tile's", ti);
+            prev_end = td.s0 + td.cn;
+            if (td.rl0 < 0 || td.rl0 + R > (int64_t)tb.rlist.size() || td.sl0 < 0 ||
+                td.sl0 + td.nsl + 1 > (int64_t)tb.tsp.size())
+                return infoli_fail(INFOLI_ERR_STATE, "layout: tile %d indexes past its tables", ti);
+            std::vector<char> seen(R, 0);
+            for (int k = 0; k < R; k++) {
+                const int r = tb.rlist[td.rl0 + k];
+                if (r >= R || seen[r]) return infoli_fail(INFOLI_ERR_STATE, "layout: tile %d's receiver list is not a permutation", ti);
+                seen[r] = 1;
+            }
+            for (int sl = 0; sl < td.nsl; sl++) {
+                const int64_t base = tb.tsp[td.sl0 + sl], span = tb.tsp[td.sl0 + sl + 1] - base;
+                if (span < 0 || span % 64 || base + span > (int64_t)tb.tcol.size())
+                    return infoli_fail(INFOLI_ERR_STATE, "layout: tile %d slice %d has a bad extent", ti, sl);
+                const int W = (int)(span / 64);
+                int md = 0;
+                for (int cw = 0; cw < CPW; cw++) {
+                    const int64_t pos = td.rl0 + (int64_t)sl * CPW + cw;
+                    const int r = tb.rlist[pos], d = tb.rdeg[pos];
+                    md = std::max(md, d);
+                    const int64_t cell = rb * R + r;
+                    // the row's junctions from this tile's senders, in row order
+                    std::vector<int64_t> es;
+                    if (cell < n)
+                        for (int64_t e = g.row_ptr[cell]; e < g.row_ptr[cell + 1]; e++)
+                            if (g.col[e] >= td.s0 && g.col[e] < td.s0 + td.cn) es.push_back(e);
+                    if ((int64_t)es.size() != d)
+                        return infoli_fail(INFOLI_ERR_STATE, "layout: tile %d receiver %lld lists %d junctions, the network %zu", ti,
+                                           (long long)cell, d, es.size());
+                    for (int p = 0; p < W * L; p++) {
+                        const int64_t slot = base + (int64_t)(p / L) * 64 + cw * L + (p % L);
+                        // every slot is read (padding too: its term is dropped), so every id stays in the chunk
+                        if (tb.tcol[slot] >= td.cn)
+                            return infoli_fail(INFOLI_ERR_STATE, "layout: tile %d slot %lld reads sender %d of a %d-cell chunk", ti,
+                                               (long long)slot, tb.tcol[slot], td.cn);
+                        if (p < d) {
+                            const int64_t e = es[p];
+                            if (td.s0 + tb.tcol[slot] != g.col[e] || tb.tw[slot] != (T)g.w[e] ||
+                                (td.wconst && (T)g.w[e] != (T)td.wc))
+                                return infoli_fail(INFOLI_ERR_STATE, "layout: tile %d receiver %lld entry %d differs from the network",
+                                                   ti, (long long)cell, p);
+                        }
+                    }
+                    covered += d;
+                }
+                if (W != (md + L - 1) / L) return infoli_fail(INFOLI_ERR_STATE, "layout: tile %d slice %d is %d wide for degree %d", ti, sl, W, md);
+            }
+        }
+    }
+    if (covered != nnz)
+        return infoli_fail(INFOLI_ERR_STATE, "layout: the tiles hold %lld junctions, the network %lld", (long long)covered, (long long)nnz);
+
+    // one step's scratch traffic
+    const int64_t slots = (int64_t)part_slots((size_t)ntiles, nrb, R);
+    std::vector<char> pw(slots, 0), dw(2 * npad, 0);
+    std::vector<int> soma(npad, 0), axon(npad, 0);
+    const auto wpart = [&](int64_t i) -> bool {
+        if (i < 0 || i >= slots) return false;
+        pw[i] = 1;
+        return true;
+    };
+    if (gather == GATHER_TILED) {
+        if (any)
+            for (int64_t ti = 0; ti < ntiles; ti++)
+                for (int k = 0; k < R; k++)
+                    if (!wpart(ti * R + tb.rlist[tb.tiles[ti].rl0 + k]))
+                        return infoli_fail(INFOLI_ERR_STATE, "layout: tile %lld writes past the partials", (long long)ti);
+        if (split == 2) {
+            // the workgroups past the tiles (k_tile_gj): group g of NT/2 cells,
+            // workgroup (tiles + j) takes groups j, j + nsa, ...
+            for (int64_t j = 0; j < nsa; j++)
+                for (int64_t gi = j; gi * (block / 2) < n; gi += nsa)
+                    for (int wv = 0; wv < block / 64; wv++)
+                        for (int lane = 0; lane < 64; lane++) {
+                            const int64_t cell = (gi * (block / 128) + (wv >> 1)) * 64 + lane;
+                            if (cell >= n) continue;
+                            if (cell >= npad) return infoli_fail(INFOLI_ERR_STATE, "layout: cell %lld past the planes", (long long)cell);
+                            if (wv & 1) {
+                                axon[cell]++;
+                                dw[cell] = dw[npad + cell] = 1;
+                            } else {
+                                soma[cell]++;
+                            }
+                        }
+            for (int64_t i = 0; i < n; i++)
+                if (soma[i] != 1 || axon[i] != 1)
+                    return infoli_fail(INFOLI_ERR_STATE, "layout: cell %lld's soma runs %d times, its axon %d (nsa %lld)",
+                                       (long long)i, soma[i], axon[i], (long long)nsa);
+        }
+    } else if (split == 1) {
+        for (int64_t rb = 0; rb < nrb; rb++)
+            for (int r = 0; r < R; r++)
+                if (!wpart(rb * R + r)) return infoli_fail(INFOLI_ERR_STATE, "layout: block %lld writes past the partials", (long long)rb);
+    }
+    const bool reads_parts = gather == GATHER_TILED || split == 1;
+    for (int64_t cell = 0; cell < n && reads_parts; cell++) {
+        const int64_t rb = cell / R, rloc = cell - rb * R;
+        const int64_t t0 = gather == GATHER_TILED ? (any ? tb.rbt0[rb] : 0) : rb;
+        const int64_t t1 = gather == GATHER_TILED ? (any ? tb.rbt0[rb + 1] : 0) : rb + 1;
+        for (int64_t tt = t0; tt < t1; tt++) {
+            const int64_t i = tt * R + rloc;
+            if (i >= slots || !pw[i])
+                return infoli_fail(INFOLI_ERR_STATE, "layout: cell %lld reads partial %lld before any kernel of the step writes it",
+                                   (long long)cell, (long long)i);
+        }
+        if (gather == GATHER_TILED && split == 2 && (!dw[cell] || !dw[npad + cell]))
+            return infoli_fail(INFOLI_ERR_STATE, "layout: cell %lld reads its dendrite sums before they are written", (long long)cell);
+    }
+    return INFOLI_OK;
+}
+
+}  // namespace
+
+extern "C" int infoli_tiled_layout_check(int64_t n, const int64_t *row_ptr, const int32_t *col, const double *w,
+                                         int precision, int gather, int lanes_per_cell, int tile_cells, int lds_kb,
+                                         int ode_split, int block, int64_t nsa)
+{
+    infoli_clear_error();
+    if (n <= 0 || n > INT32_MAX || !row_ptr || row_ptr[0] != 0)
+        return infoli_fail(INFOLI_ERR_ARG, "infoli_tiled_layout_check: bad network");
+    if (precision < INFOLI_FP64 || precision > INFOLI_FP32_KAHAN || (gather != GATHER_TILED && gather != GATHER_SWEEP) ||
+        lidx(lanes_per_cell) < 0 || tile_cells < 64 || tile_cells > 16384 || tile_cells % 64 || lds_kb < 1 ||
+        lds_kb > 156 || ode_split < 0 || ode_split > 2 || (gather == GATHER_SWEEP && ode_split == 2) ||
+        (block != 256 && block != 512 && block != 1024) || (gather == GATHER_SWEEP && block != 1024) ||
+        (ode_split == 2 && nsa < 1))
+        return infoli_fail(INFOLI_ERR_ARG, "infoli_tiled_layout_check: bad shape");
+    for (int64_t i = 0; i < n; i++)
+        if (row_ptr[i + 1] < row_ptr[i]) return infoli_fail(INFOLI_ERR_ARG, "infoli_tiled_layout_check: row_ptr decreases");
+    const int64_t nnz = row_ptr[n];
+    if (nnz > 0 && (!col || !w)) return infoli_fail(INFOLI_ERR_ARG, "infoli_tiled_layout_check: null col or w");
+    for (int64_t e = 0; e < nnz; e++)
+        if (col[e] < 0 || col[e] >= n) return infoli_fail(INFOLI_ERR_ARG, "infoli_tiled_layout_check: col[%lld] out of range", (long long)e);
+    const CsrView g{n, row_ptr, col, w};
+    const size_t tsize = precision == INFOLI_FP64 ? 8 : 4;
+    const size_t psz = precision == INFOLI_FP32_KAHAN ? 8 : tsize;
+    return precision == INFOLI_FP64
+               ? check_tiled<double>(g, tsize, psz, gather, lanes_per_cell, tile_cells, lds_kb, ode_split, block, nsa)
+               : check_tiled<float>(g, tsize, psz, gather, lanes_per_cell, tile_cells, lds_kb, ode_split, block, nsa);
 }

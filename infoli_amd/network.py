@@ -105,6 +105,7 @@ def library():
         "infoli_read_conn_triplets": ([ctypes.c_char_p, i64, d, P, P, P, P], i32),
         "infoli_read_values": ([ctypes.c_char_p, i64, P], i32),
         "infoli_read_iapp_file": ([ctypes.c_char_p, i64, i64, P, P], i32),
+        "infoli_tiled_layout_check": ([i64, P, P, P, i32, i32, i32, i32, i32, i32, i32, i64], i32),
         "infoli_last_error": ([], ctypes.c_char_p),
         "infoli_strerror": ([i32], ctypes.c_char_p),
         "infoli_version": ([], ctypes.c_char_p),
@@ -375,6 +376,20 @@ def gen_symmetric(seed, n, p, w_lo, w_hi=None):
     L = library()
     return _two_call(L.infoli_gen_symmetric, n, int(seed), int(n), float(p), float(w_lo),
                      float(w_hi))
+
+
+def tiled_layout_check(row_ptr, col, w, precision="fp32k", gather=GATHER_TILED, lanes_per_cell=1,
+                       tile_cells=512, lds_kb=2, ode_split=0, block=1024, nsa=1):
+    """Host-only check of the tiled layout the library builds for this
+    network and shape, and of one step's scratch reads and writes; raises
+    InfoliError naming the first violation."""
+    row_ptr = np.ascontiguousarray(row_ptr, np.int64)
+    col = np.ascontiguousarray(col, np.int32)
+    w = np.ascontiguousarray(w, np.float64)
+    n = len(row_ptr) - 1
+    _check(library().infoli_tiled_layout_check(n, _p(row_ptr), _p(col), _p(w), _prec(precision), int(gather),
+                                               int(lanes_per_cell), int(tile_cells), int(lds_kb),
+                                               int(ode_split), int(block), int(nsa)), "tiled_layout_check")
 
 
 def gen_directed(seed, n, p, w0):
