@@ -122,9 +122,68 @@ def cpu_baseline(wl, target_s=18.0):
     return ({"value": n * k / el, "unit": "cell-steps/s", "cores": threads, "kind": "port",
              "sample": f"{wl['name']}: steps [0, {k}) of {wl['steps']}, all {n} cells, FP32 "
                        f"OpenMP restatement of src/openmp/infoli.c:496-679 (oracle/infoli_oracle.c, "
-                       f"gcc -O3 -march=x86-64-v3), {threads} threads on {cpu}, {binding}; median of "
-                       f"3 runs: {', '.join(f'{r:.2f}' for r in runs)} s"},
+                       f"gcc -O3 -march=x86-64-v3), {threads} OpenMP threads = the logical CPUs in this "
+                       f"process's affinity mask (not the host's physical core count), on {cpu}, "
+                       f"{binding}; median of 3 runs: {', '.join(f'{r:.2f}' for r in runs)} s"},
             n * k / el64)
+
+
+def graph_steps(steps, default):
+    """Steps per graph replay for a timed run of `steps`: the largest divisor
+    of it not above `default`, so the timed steps are whole replays of one
+    graph (captured before the timed region)."""
+    return max(d for d in range(1, min(steps, default) + 1) if steps % d == 0)
+
+
+def layout_bytes_per_step(n, nnz, prec, constant_w):
+    """Bytes one step moves in the tiled layout itself: per neighbour entry a
+    16-bit chunk-local sender id, plus the conductance unless the tile's
+    conductance is constant (then none); per cell the 144 (FP32) or 288 B of
+    state, stimulus and V_dend of the algorithmic count."""
+    t = 8 if prec == "fp64" else 4
+    return (2 + (0 if constant_w else t)) * nnz + 36 * t * n
+
+
+def batched_leg(I, W, steps, warmup, per_net_steps, per_net_warmup, with_fp64=True):
+    """batched_512 in FP32 with compensated sums: all 512 networks in one
+    context (block-diagonal), 500-step graph replays; beside it the same
+    networks one after another, each in its own FP64 context."""
+    wl = W.batched_512()
+    n, nnz = wl["n"], wl.nnz
+    net = I.Network(n, precision="fp32k")
+    W.load_into(net, wl)
+    G = graph_steps(steps, 500)
+    net.set_exec(steps_per_launch=G)
+    net.prepare(steps, 100)
+    net.advance(warmup, trace_stride=100)
+    net.synchronize()
+    t0 = time.perf_counter()
+    net.advance(steps, trace_stride=100)
+    net.synchronize()
+    el = time.perf_counter() - t0
+    dev_ms = net.last_timing()["advance_ms"]
+    launches = net.last_launches()
+    exec_cfg = net.exec
+    net.close()
+    value = n * steps / el
+    step_ms = dev_ms / steps
+    lb = layout_bytes_per_step(n, nnz, "fp32k", constant_w=True)
+    out = {"workload": "batched_512", "networks": len(wl["partition"]) - 1, "cells": n,
+           "neighbour_entries": nnz, "steps": steps, "warmup": warmup,
+           "cell_steps_per_s": value, "ms_per_step": el * 1e3 / steps, "exec": exec_cfg,
+           "launch_path": dict(launches, steps_per_graph=G),
+           "roofline": {"bound": "hbm", "bytes_per_step": lb, "achieved": lb / (step_ms * 1e-3) / 1e9,
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": lb / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                        "note": "bytes of the tiled layout itself: 2 B per entry (16-bit ids; the "
+                                "conductance 0.04 is constant per tile and not stored) + 144 B per cell"}}
+    if with_fp64:
+        print("# batched_512: per-network FP64 launches", file=sys.stderr, flush=True)
+        v, desc = per_network_fp64(I, wl, per_net_steps, per_net_warmup)
+        out["per_network_fp64_cell_steps_per_s"] = v
+        out["per_network_fp64"] = desc
+        out["batched_over_per_network_fp64"] = value / v
+    return out
 
 
 def per_network_fp64(I, wl, steps, warmup):
@@ -140,6 +199,8 @@ def per_network_fp64(I, wl, steps, warmup):
         net.set_connectivity(rp[a:b + 1] - e0, col[e0:e1] - a, w[e0:e1])
         net.set_gcal(wl["gcal"][a:b])
         net.set_iapp(wl["iapp_base"][a:b])
+        net.set_exec(steps_per_launch=graph_steps(steps, 500))
+        net.prepare(steps)
         net.advance(warmup)
         net.synchronize()
         t0 = time.perf_counter()
@@ -175,6 +236,8 @@ def main():
                          "plain against FP64, max |dV| over 100 ms and spike times after it")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched_512 leg (all networks in one context vs per-network FP64)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -197,9 +260,8 @@ def main():
             net.set_exec(lds_kb=args.lds_kb)
         if args.tile_cells:
             net.set_exec(tile_cells=args.tile_cells)
-        spl = args.steps_per_launch or (500 if wl.get("partition") is not None else 0)
-        if spl:
-            net.set_exec(steps_per_launch=spl)
+        spl = args.steps_per_launch or graph_steps(args.steps, 500 if wl.get("partition") is not None else 100)
+        net.set_exec(steps_per_launch=spl)
         if args.gather != "auto" or args.ode_split >= 0:
             g = {"direct": 0, "lds": 1, "staged": 2, "tiled": 3, "sweep": 4}.get(args.gather)
             net.set_exec(gather=g, ode_split=args.ode_split if args.ode_split >= 0 else None)
@@ -214,6 +276,9 @@ def main():
         return net
 
     def timed(net, steps, warmup):
+        # the graph the timed steps replay is captured here, before the
+        # warm-up and the timed region
+        net.prepare(steps, args.stride)
         if warmup:
             net.advance(warmup, trace_stride=args.stride)
         net.synchronize()
@@ -243,6 +308,7 @@ def main():
     el = timed(net, args.steps, args.warmup)
     value = n * args.steps / el
     dev_ms = net.last_timing()["advance_ms"]
+    launches = dict(net.last_launches(), steps_per_graph=net.exec["steps_per_launch"])
 
     # Kernel timing: HIP events recorded on the library's stream around the
     # timed region (infoli_last_timing); one step's launches (the tiled
@@ -252,6 +318,12 @@ def main():
     mean_launch_ms = dev_ms / args.steps
     steps_per_launch = 1.0
     bytes_per_launch = algorithmic_bytes_per_step(n, nnz, args.precision)
+    bytes_note = "8 B per neighbour entry (int32 id + conductance) + 144 B per cell"
+    if wl.get("partition") is not None and exec_cfg["gather"] == 3:
+        # the batched networks' tiles hold 16-bit ids and no conductances
+        # (w = 0.04 everywhere): the step is priced against its own layout
+        bytes_per_launch = layout_bytes_per_step(n, nnz, args.precision, constant_w=True)
+        bytes_note = "tiled layout's own bytes: 2 B per entry (16-bit ids, constant conductance) + 144 B per cell"
     achieved = bytes_per_launch / (mean_launch_ms * 1e-3) / 1e9
     net.set_profiling(True)
     prof_steps = min(args.steps, 2000)
@@ -265,7 +337,7 @@ def main():
     tr = measured_traffic(kname, wl["name"], args.precision)
     traffic = None if tr is None else tr * steps_per_launch
     exp_rate = nnz / (mean_launch_ms * 1e-3)
-    extra = {"device_ms": round(dev_ms, 3), "exec": exec_cfg,
+    extra = {"device_ms": round(dev_ms, 3), "exec": exec_cfg, "launch_path": launches,
              "mean_step_launches_ms": round(mean_launch_ms, 5),
              "per_step_bracketed_ms": round(per_step_bracketed_ms, 5),
              "exp_rate": {"achieved": exp_rate, "peak": VEXP_PEAK_PER_S, "unit": "exp/s",
@@ -286,6 +358,11 @@ def main():
         extra["per_network_fp64_cell_steps_per_s"] = v
         extra["per_network_fp64"] = desc
         extra["batched_over_per_network_fp64"] = value / v
+    elif not args.no_batched:
+        # a bounded batched leg beside the headline: 1,000 steps (two 500-step
+        # graph replays) after 500, against 500 per-network FP64 steps each
+        print("# batched_512 leg", file=sys.stderr, flush=True)
+        extra["batched"] = batched_leg(I, W, 1000, 500, 500, 100, with_fp64=not args.no_fp64)
 
     cpu = None
     if not args.no_cpu:
@@ -314,7 +391,7 @@ def main():
                    "trace_stride": args.stride, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "bytes_per_launch": bytes_per_launch,
+                     "bytes_per_launch": bytes_per_launch, "bytes_counted": bytes_note,
                      "launch": "one step (its kernels, back to back in the graph)",
                      "kernel": kname},
         "cpu_baseline": cpu,

@@ -208,6 +208,14 @@ int infoli_gen_rand_init(uint64_t seed, int64_t n, double delta, double *g_cal, 
 int infoli_advance(infoli_t *ctx, int64_t n_steps, int64_t trace_stride,
                    double *trace);
 
+/* Build now everything a following infoli_advance(ctx, n_steps,
+ * trace_stride, ...) needs -- the neighbour layout of the execution
+ * strategy, the device trace ring and pinned staging (trace_stride 0: no
+ * trace), the hipGraph it replays -- without running a step, so that the
+ * advance's time is the steps' alone.  No reference counterpart: the
+ * reference's setup before its timer (src/openmp/infoli.c:204-428). */
+int infoli_prepare(infoli_t *ctx, int64_t n_steps, int64_t trace_stride);
+
 /* Number of records infoli_advance(ctx, n_steps, stride, ...) will write. */
 int64_t infoli_trace_records(const infoli_t *ctx, int64_t n_steps,
                              int64_t trace_stride);
@@ -275,6 +283,13 @@ int infoli_autotune(infoli_t *ctx, int64_t trial_steps, infoli_exec *chosen);
  * stream), and of its dominant kernel launches: total ms and launch count. */
 int infoli_last_timing(const infoli_t *ctx, double *advance_ms,
                        double *kernel_ms, int64_t *kernel_launches);
+
+/* How the last infoli_advance launched its steps: hipGraph replays of
+ * steps_per_launch steps each, steps launched one by one (profiling, step
+ * mode, or a remainder shorter than a graph), and graphs captured during
+ * the call (0 when infoli_prepare or an earlier call captured them). */
+int infoli_last_launches(const infoli_t *ctx, int64_t *graph_replays,
+                         int64_t *direct_steps, int64_t *captures);
 
 /* Bracket every step-kernel launch with HIP events (per-step launches
  * instead of graph replays while on) so that infoli_last_timing reports

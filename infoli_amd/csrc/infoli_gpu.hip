@@ -134,6 +134,9 @@ struct infoli_ctx {
     bool profiling = false;
     double last_ms = 0, kern_ms = 0;
     int64_t kern_launches = 0;
+    // launches of the last infoli_advance: graph replays, steps launched
+    // one by one, graph captures
+    int64_t n_replays = 0, n_direct = 0, n_captures = 0;
 
     int num_cus = 0;
 };
@@ -1760,6 +1763,7 @@ int capture_graph(infoli_ctx *c, const StepArgs<T> &a, int mode, int G, int grid
         memcmp(c->gkey.args, key.args, key.args_len) == 0)
         return INFOLI_OK;
     drop_graph(c);
+    c->n_captures++;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int rc = INFOLI_OK;
     if (mode == INFOLI_EXEC_GRAPH) {
@@ -1784,6 +1788,25 @@ int capture_graph(infoli_ctx *c, const StepArgs<T> &a, int mode, int G, int grid
     return INFOLI_OK;
 }
 
+/* Capture now the graph that run_steps replays for a piece of `count`
+ * steps (graph and persistent modes; nothing to do for fewer steps than one
+ * launch holds, or while profiling, when steps are launched one by one).
+ * Cached: a later call with the same strategy and arguments reuses it. */
+template <typename T>
+int prepare_graph(infoli_ctx *c, StepArgs<T> a, int64_t count)
+{
+    const int G = c->exec.steps_per_launch;
+    if (c->profiling || count < G) return INFOLI_OK;
+    a.d_step = c->d_step;
+    if (c->exec.mode == INFOLI_EXEC_GRAPH) return capture_graph(c, a, INFOLI_EXEC_GRAPH, G, 0);
+    if (c->exec.mode == INFOLI_EXEC_PERSISTENT && !tiled_layout(c->exec)) {
+        const int grid = persist_grid(c);
+        if (grid <= 0) return grid < 0 ? grid : infoli_fail(INFOLI_ERR_HIP, "no persistent grid");
+        return capture_graph(c, a, INFOLI_EXEC_PERSISTENT, G, grid);
+    }
+    return INFOLI_OK;
+}
+
 /* steps [t, t+count) in the context's execution mode */
 template <typename T>
 int run_steps(infoli_ctx *c, StepArgs<T> a, int64_t t, int64_t count)
@@ -1792,6 +1815,7 @@ int run_steps(infoli_ctx *c, StepArgs<T> a, int64_t t, int64_t count)
     if (c->profiling && mode == INFOLI_EXEC_GRAPH) mode = INFOLI_EXEC_STEP;
     if (mode == INFOLI_EXEC_STEP) {
         a.d_step = nullptr;
+        c->n_direct += count;
         for (int64_t k = 0; k < count; k++) {
             RCHK(prof_begin(c));
             RCHK(launch_step(c, &a, t + k));
@@ -1806,10 +1830,14 @@ int run_steps(infoli_ctx *c, StepArgs<T> a, int64_t t, int64_t count)
     HIPCHK(hipGetLastError());
     int64_t done = 0;
     if (mode == INFOLI_EXEC_GRAPH) {
-        if (count >= G) RCHK(capture_graph(c, a, mode, G, 0));
-        for (; done + G <= count; done += G) HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+        RCHK(prepare_graph(c, a, count));
+        for (; done + G <= count; done += G) {
+            HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+            c->n_replays++;
+        }
         StepArgs<T> b = a;
         b.d_step = nullptr;
+        c->n_direct += count - done;
         for (; done < count; done++) RCHK(launch_step(c, &b, t + done));
         return INFOLI_OK;
     }
@@ -1818,12 +1846,14 @@ int run_steps(infoli_ctx *c, StepArgs<T> a, int64_t t, int64_t count)
         return infoli_fail(INFOLI_ERR_ARG, "tiled gather has no persistent form");
     int grid = persist_grid(c);
     if (grid <= 0) return grid < 0 ? grid : infoli_fail(INFOLI_ERR_HIP, "no persistent grid");
-    if (!c->profiling && count >= G) RCHK(capture_graph(c, a, mode, G, grid));
+    RCHK(prepare_graph(c, a, count));
     while (done < count) {
         const int piece = (int)std::min<int64_t>(G, count - done);
         if (!c->profiling && piece == G && c->gexec) {
             HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+            c->n_replays++;
         } else {
+            c->n_direct += piece;
             RCHK(prof_begin(c));
             RCHK(launch_persist(c, &a, piece, grid));
             RCHK(prof_end(c));
@@ -1872,33 +1902,56 @@ int drain_records(infoli_ctx *c, int buf, int64_t per_buf, int64_t nrec, double 
     return INFOLI_OK;
 }
 
+/* The device trace ring and the pinned staging for an advance of n_steps
+ * with a record every stride_eff steps (0: no trace); *per_piece = records
+ * per piece (the pieces' copies and widening overlap the next piece).  A
+ * call that records at most one step needs one slot; any other gets a whole
+ * piece of kTracePieceSteps steps' records, so calls of different lengths
+ * share one ring and the graph captured with it.  The ring only grows. */
+template <typename T>
+int prepare_trace(infoli_ctx *c, int64_t n_steps, int64_t stride_eff, int64_t *per_piece)
+{
+    *per_piece = 1;
+    if (stride_eff <= 0) return INFOLI_OK;
+    const int64_t n = c->n;
+    const int64_t want = infoli_trace_records(c, n_steps, stride_eff);
+    const int64_t budget = std::max<int64_t>(1, (int64_t)(kTraceRingBytes / ((size_t)n * sizeof(T))));
+    const int64_t piece = std::max<int64_t>(1, (kTracePieceSteps + stride_eff - 1) / stride_eff);
+    const int64_t cap = want <= 1 ? 1 : std::min(piece, budget);
+    RCHK(ensure_stage(c, (size_t)2 * cap * n * sizeof(T)));
+    if (cap > c->trace_cap) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (c->d_trace) HIPCHK(hipFree(c->d_trace));
+        c->d_trace = nullptr;
+        c->trace_cap = 0;
+        HIPCHK(hipMalloc(&c->d_trace, (size_t)cap * n * sizeof(T)));
+        c->trace_cap = cap;
+        drop_graph(c);
+    }
+    *per_piece = cap;
+    return INFOLI_OK;
+}
+
+template <typename T>
+int prepare_impl(infoli_ctx *c, int64_t n_steps, int64_t stride_eff)
+{
+    int64_t per_piece = 1;
+    RCHK(prepare_trace<T>(c, n_steps, stride_eff, &per_piece));
+    const int64_t first = stride_eff > 0 ? std::min<int64_t>(n_steps, per_piece * stride_eff) : n_steps;
+    return prepare_graph<T>(c, make_args<T>(c, stride_eff), first);
+}
+
 template <typename T>
 int advance_impl(infoli_ctx *c, int64_t n_steps, int64_t stride, double *trace)
 {
     const int64_t n = c->n;
     const int64_t stride_eff = trace ? stride : 0;
-    int64_t cap = 1;
-    int64_t per_piece = 1;   // records per piece: the pieces' copies and widening overlap the next piece
-    if (stride_eff > 0) {
-        const int64_t want = infoli_trace_records(c, n_steps, stride_eff);
-        const int64_t budget = std::max<int64_t>(1, (int64_t)(kTraceRingBytes / ((size_t)n * sizeof(T))));
-        per_piece = std::max<int64_t>(1, (kTracePieceSteps + stride_eff - 1) / stride_eff);
-        cap = std::max<int64_t>(1, std::min(std::min(want, per_piece), budget));
-        per_piece = cap;
-        RCHK(ensure_stage(c, (size_t)2 * cap * n * sizeof(T)));
-        if (cap > c->trace_cap) {
-            HIPCHK(hipStreamSynchronize(c->stream));
-            if (c->d_trace) HIPCHK(hipFree(c->d_trace));
-            c->d_trace = nullptr;
-            c->trace_cap = 0;
-            HIPCHK(hipMalloc(&c->d_trace, (size_t)cap * n * sizeof(T)));
-            c->trace_cap = cap;
-            drop_graph(c);
-        }
-    }
+    int64_t per_piece = 1;
+    RCHK(prepare_trace<T>(c, n_steps, stride_eff, &per_piece));
     StepArgs<T> a = make_args<T>(c, stride_eff);
     c->kern_ms = 0;
     c->kern_launches = 0;
+    c->n_replays = c->n_direct = c->n_captures = 0;
     for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
     c->prof_ev.clear();
 
@@ -1967,6 +2020,18 @@ extern "C" int infoli_advance(infoli_t *c, int64_t n_steps, int64_t trace_stride
                      : advance_impl<double>(c, n_steps, trace_stride, trace);
 }
 
+extern "C" int infoli_prepare(infoli_t *c, int64_t n_steps, int64_t trace_stride)
+{
+    infoli_clear_error();
+    RCHK(validate_ctx(c, "infoli_prepare"));
+    if (n_steps < 0) return infoli_fail(INFOLI_ERR_ARG, "infoli_prepare: n_steps < 0");
+    if (trace_stride < 0) return infoli_fail(INFOLI_ERR_ARG, "infoli_prepare: trace_stride < 0");
+    if (n_steps == 0) return INFOLI_OK;
+    HIPCHK(hipSetDevice(c->device));
+    RCHK(ensure_layout(c));
+    return is_f32(c) ? prepare_impl<float>(c, n_steps, trace_stride) : prepare_impl<double>(c, n_steps, trace_stride);
+}
+
 extern "C" int infoli_gap_currents(infoli_t *c, double *i_c)
 {
     infoli_clear_error();
@@ -2004,6 +2069,16 @@ extern "C" int infoli_last_timing(const infoli_t *c, double *advance_ms, double 
     if (advance_ms) *advance_ms = c->last_ms;
     if (kernel_ms) *kernel_ms = c->kern_ms;
     if (kernel_launches) *kernel_launches = c->kern_launches;
+    return INFOLI_OK;
+}
+
+extern "C" int infoli_last_launches(const infoli_t *c, int64_t *graph_replays, int64_t *direct_steps,
+                                    int64_t *captures)
+{
+    RCHK(validate_ctx(c, "infoli_last_launches"));
+    if (graph_replays) *graph_replays = c->n_replays;
+    if (direct_steps) *direct_steps = c->n_direct;
+    if (captures) *captures = c->n_captures;
     return INFOLI_OK;
 }
 

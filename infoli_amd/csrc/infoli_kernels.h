@@ -185,7 +185,9 @@ __device__ __forceinline__ float ode_dend_pre(const Model<float> &P, CellState32
     c.q = P.DELTA * dq_dt + c.q;
     // Dend Ca current OMP:567-572
     float alpha_r = D<FM>(1.7f, 1 + E<FM>(D<FM>(-(Vd - 5), 13.9f)));
-    float beta_r = D<FM>(0.02f * (Vd + 8.5f), E<FM>(D<FM>(Vd + 8.5f, 5)) - 1);
+    // 0/0 at Vd = -8.5 exactly (the reference computes NaN there): the limit 0.02 * 5
+    const float xr = Vd + 8.5f;
+    float beta_r = xr == 0 ? 0.1f : D<FM>(0.02f * xr, E<FM>(D<FM>(xr, 5)) - 1);
     float r_inf = D<FM>(alpha_r, alpha_r + beta_r);
     float tau_r = D<FM>(5, alpha_r + beta_r);
     float dr_dt = D<FM>(r_inf - c.r, tau_r);
@@ -259,7 +261,9 @@ __device__ __forceinline__ float ode_soma(const Model<float> &P, CellState32 &c,
     c.n = P.DELTA * dn_dt + c.n;
     c.p = P.DELTA * dp_dt + c.p;
     // Soma potassium X OMP:627-632 (literal 0.05f)
-    float alpha_x_s = D<FM>(0.13f * (Vs + 25), 1 - E<FM>(D<FM>(-(Vs + 25), 10)));
+    // 0/0 at Vs = -25 exactly (NaN in the reference): the limit 0.13 * 10
+    const float xs = Vs + 25;
+    float alpha_x_s = xs == 0 ? 1.3f : D<FM>(0.13f * xs, 1 - E<FM>(D<FM>(-xs, 10)));
     float beta_x_s = 1.69f * E<FM>(-0.0125f * (Vs + 35));
     float x_inf_s = D<FM>(alpha_x_s, alpha_x_s + beta_x_s);
     float tau_x_s = D<FM>(1, alpha_x_s + beta_x_s);
@@ -291,7 +295,8 @@ __device__ __forceinline__ float ode_axon(const Model<float> &P, CellState32 &c,
     c.ma = m_inf_a;
     c.ha = c.ha + P.DELTA * dh_dt_a;
     // Axon potassium OMP:656-661
-    float alpha_x_a = D<FM>(0.13f * (Va + 25), 1 - E<FM>(D<FM>(-(Va + 25), 10)));
+    const float xa = Va + 25;   // 0/0 at -25 exactly: the limit
+    float alpha_x_a = xa == 0 ? 1.3f : D<FM>(0.13f * xa, 1 - E<FM>(D<FM>(-xa, 10)));
     float beta_x_a = 1.69f * E<FM>(-0.0125f * (Va + 35));
     float x_inf_a = D<FM>(alpha_x_a, alpha_x_a + beta_x_a);
     float tau_x_a = D<FM>(1, alpha_x_a + beta_x_a);
@@ -323,7 +328,8 @@ __device__ __forceinline__ double ode_dend_pre(const Model<double> &P, CellState
     double q = P.DELTA * dq_dt + c.q;
     // DendCaCurr ST:449-468
     double alpha_r = 1.7 / (1 + exp(-(prevV_dend - 5) / 13.9));
-    double beta_r = 0.02 * (prevV_dend + 8.5) / (exp((prevV_dend + 8.5) / 5) - 1);
+    const double xr = prevV_dend + 8.5;   // 0/0 at -8.5 exactly: the limit 0.02 * 5
+    double beta_r = xr == 0 ? 0.1 : 0.02 * xr / (exp(xr / 5) - 1);
     double r_inf = alpha_r / (alpha_r + beta_r);
     double tau_r = 5 / (alpha_r + beta_r);
     double dr_dt = (r_inf - c.r) / tau_r;
@@ -393,7 +399,8 @@ __device__ __forceinline__ double ode_soma(const Model<double> &P, CellState64 &
     double n = P.DELTA * dn_dt + c.n;
     double p = P.DELTA * dp_dt + c.p;
     // SomaPotassiumX ST:690-709 (literal 0.05)
-    double alpha_x_s = 0.13 * (prevV_soma + 25) / (1 - exp(-(prevV_soma + 25) / 10));
+    const double xs = prevV_soma + 25;    // 0/0 at -25 exactly: the limit 0.13 * 10
+    double alpha_x_s = xs == 0 ? 1.3 : 0.13 * xs / (1 - exp(-xs / 10));
     double beta_x_s = 1.69 * exp(-0.0125 * (prevV_soma + 35));
     double x_inf_s = alpha_x_s / (alpha_x_s + beta_x_s);
     double tau_x_s = 1 / (alpha_x_s + beta_x_s);
@@ -425,7 +432,8 @@ __device__ __forceinline__ double ode_axon(const Model<double> &P, CellState64 &
     double m_a = m_inf_a;
     double h_a = c.ha + P.DELTA * dh_dt_a;
     // AxonPotassium ST:812-831
-    double alpha_x_a = 0.13 * (prevV_axon + 25) / (1 - exp(-(prevV_axon + 25) / 10));
+    const double xa = prevV_axon + 25;
+    double alpha_x_a = xa == 0 ? 1.3 : 0.13 * xa / (1 - exp(-xa / 10));
     double beta_x_a = 1.69 * exp(-0.0125 * (prevV_axon + 35));
     double x_inf_a = alpha_x_a / (alpha_x_a + beta_x_a);
     double tau_x_a = 1 / (alpha_x_a + beta_x_a);

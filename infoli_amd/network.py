@@ -88,6 +88,8 @@ def library():
         "infoli_get_state": ([P, i32, P], i32),
         "infoli_advance": ([P, i64, i64, P], i32),
         "infoli_trace_records": ([P, i64, i64], i64),
+        "infoli_prepare": ([P, i64, i64], i32),
+        "infoli_last_launches": ([P, P, P, P], i32),
         "infoli_step_count": ([P], i64),
         "infoli_gap_currents": ([P, P], i32),
         "infoli_get_exec": ([P, P], i32),
@@ -312,6 +314,19 @@ class Network:
         if rc != 0 and not (allow_nonfinite and rc == ERR_NONFINITE):
             _check(rc, "advance")
         return tr
+
+    def prepare(self, n_steps, trace_stride=None):
+        """Build the layout, trace ring and graph a following
+        advance(n_steps, trace_stride) uses, without running a step."""
+        _check(library().infoli_prepare(self._h, int(n_steps), int(trace_stride or 0)), "prepare")
+
+    def last_launches(self):
+        """How the last advance launched: graph replays, steps launched one
+        by one, graphs captured during the call."""
+        r, d, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _check(library().infoli_last_launches(self._h, ctypes.byref(r), ctypes.byref(d), ctypes.byref(c)),
+               "last_launches")
+        return dict(graph_replays=r.value, direct_steps=d.value, captures=c.value)
 
     def gap_currents(self):
         out = np.zeros(self.n, np.float64)

@@ -18,6 +18,17 @@ run see bit-identical inputs.  Streams: 0 graph, 1 w, 2 g_CaL, 3 Iapp,
                Iapp ~ U[-1, 3] per network, 20,000 steps.
   large_1m     1,000,000 cells, G(n, 50/(n-1)), w ~ U[0.02, 0.06],
                g_CaL ~ U[0.5, 1.1], Iapp ~ N(1, 1) constant, 2,000 steps.
+
+And one network for the precision statement where the gap junctions act
+on spikes (the five above either stay subthreshold or, in plain_1k and
+batched_512, keep every cell of a network identical, so I_c = 0):
+
+  spiking_20k  20,000 cells, G(n, 50/(n-1)), w ~ U[0.02, 0.06],
+               g_CaL ~ U[0.9, 1.4], Iapp ~ U[-1.2, -0.6] constant, 6,000
+               steps.  Isolated cells of these parameters fire
+               persistently; coupled, they fire from about 100 ms on, at
+               different times, so every spike drives current through its
+               junctions.
 """
 import math
 
@@ -28,7 +39,7 @@ from .network import gen_mask, gen_normal, gen_symmetric, gen_uniform
 MASTER_SEED = 20261203
 S_GRAPH, S_W, S_GCAL, S_IAPP, S_MASK, S_SIZES = 0, 1, 2, 3, 4, 5
 
-NAMES = ("plain_1k", "dense_10k", "sparse_100k", "batched_512", "large_1m")
+NAMES = ("plain_1k", "dense_10k", "sparse_100k", "batched_512", "large_1m", "spiking_20k")
 
 
 def _net_seed(seed, k):
@@ -101,8 +112,16 @@ def large_1m(n=1000000, steps=2000, seed=MASTER_SEED, mean_deg=50.0):
                     t0=0, t1=0, steps=steps, partition=None)
 
 
+def spiking_20k(n=20000, steps=6000, seed=MASTER_SEED, mean_deg=50.0):
+    rp, col, w = gen_symmetric(seed, n, mean_deg / (n - 1), 0.02, 0.06)
+    return Workload(name="spiking_20k", n=n, row_ptr=rp, col=col, w=w,
+                    gcal=gen_uniform(seed, S_GCAL, n, 0.9, 1.4),
+                    iapp_base=gen_uniform(seed, S_IAPP, n, -1.2, -0.6), mask=None, amp=0.0,
+                    t0=0, t1=0, steps=steps, partition=None)
+
+
 BUILDERS = {"plain_1k": plain_1k, "dense_10k": dense_10k, "sparse_100k": sparse_100k,
-            "batched_512": batched_512, "large_1m": large_1m}
+            "batched_512": batched_512, "large_1m": large_1m, "spiking_20k": spiking_20k}
 
 
 def build(name, **kw):

@@ -34,7 +34,11 @@
  *   - gap-junction sums match a long-double brute force, are odd in
  *     V_i - V_j and conserve sum_i I_c = 0 for symmetric w;
  *   - under a degree budget, FP32K cells at or above it equal the FP64
- *     sum (float)gj64 bit for bit on float-exact inputs.
+ *     sum (float)gj64 bit for bit on float-exact inputs;
+ *   - at the removable singularities of beta_r (V_dend = -8.5) and alpha_x
+ *     (V = -25), where the reference computes 0/0 = NaN, the limit (0.1 and
+ *     1.3) is used instead: the one-step result at the singular voltage lies
+ *     between those at the neighbouring floats/doubles (continuity).
  * Every exported function is pinned by at least one of these.
  */
 #include <math.h>
@@ -103,7 +107,9 @@ static void cell64(const p64 *P, double **s, int64_t i, double I_c, double I_app
 
     /* DendCaCurr ST:449-468 */
     double alpha_r = 1.7 / (1 + exp(-(prevV_dend - 5) / 13.9));
-    double beta_r = 0.02 * (prevV_dend + 8.5) / (exp((prevV_dend + 8.5) / 5) - 1);
+    /* removable singularity (0/0 at V = -8.5 exactly): its limit 0.02 * 5 */
+    const double xr = prevV_dend + 8.5;
+    double beta_r = xr == 0 ? 0.1 : 0.02 * xr / (exp(xr / 5) - 1);
     double r_inf = alpha_r / (alpha_r + beta_r);
     double tau_r = 5 / (alpha_r + beta_r);
     double dr_dt = (r_inf - s[CAL_R][i]) / tau_r;
@@ -158,7 +164,9 @@ static void cell64(const p64 *P, double **s, int64_t i, double I_c, double I_app
     double p = P->DELTA * dp_dt + s[POT_P][i];
 
     /* SomaPotassiumX ST:690-709 (literal 0.05, not DELTA) */
-    double alpha_x_s = 0.13 * (prevV_soma + 25) / (1 - exp(-(prevV_soma + 25) / 10));
+    /* removable singularity (0/0 at V = -25 exactly): its limit 0.13 * 10 */
+    const double xs = prevV_soma + 25;
+    double alpha_x_s = xs == 0 ? 1.3 : 0.13 * xs / (1 - exp(-xs / 10));
     double beta_x_s = 1.69 * exp(-0.0125 * (prevV_soma + 35));
     double x_inf_s = alpha_x_s / (alpha_x_s + beta_x_s);
     double tau_x_s = 1 / (alpha_x_s + beta_x_s);
@@ -185,7 +193,8 @@ static void cell64(const p64 *P, double **s, int64_t i, double I_c, double I_app
     double h_a = s[NA_HA][i] + P->DELTA * dh_dt_a;
 
     /* AxonPotassium ST:812-831 */
-    double alpha_x_a = 0.13 * (prevV_axon + 25) / (1 - exp(-(prevV_axon + 25) / 10));
+    const double xa = prevV_axon + 25;
+    double alpha_x_a = xa == 0 ? 1.3 : 0.13 * xa / (1 - exp(-xa / 10));
     double beta_x_a = 1.69 * exp(-0.0125 * (prevV_axon + 35));
     double x_inf_a = alpha_x_a / (alpha_x_a + beta_x_a);
     double tau_x_a = 1 / (alpha_x_a + beta_x_a);
@@ -296,7 +305,7 @@ static float gj32k_cell(const float *vd, int64_t i, const int64_t *rp,
 static void cell32(const p32 *P, float **s, int64_t i, float I_c, float iApp)
 {
     float *V_dend = s[V_DEND], *V_soma = s[V_SOMA], *V_axon = s[V_AXON];
-    float q_inf, tau_q, dq_dt, alpha_r, beta_r, r_inf, tau_r, dr_dt;
+    float q_inf, tau_q, dq_dt, alpha_r, beta_r, r_inf, tau_r, dr_dt, xr, xs, xa;
     float alpha_s, beta_s, s_inf, tau_s, ds_dt, dCa_dt;
     float I_sd, I_CaH_temp, I_K_Ca, I_ld, I_h, dVd_dt;
     float k_inf, l_inf, tau_k, tau_l, dk_dt, dl_dt, m_inf, h_inf, tau_h, dh_dt;
@@ -314,7 +323,8 @@ static void cell32(const p32 *P, float **s, int64_t i, float I_c, float iApp)
 
     /* Dend Ca current OMP:567-572 */
     alpha_r = 1.7f / (1 + expf(-(V_dend[i] - 5) / 13.9f));
-    beta_r = 0.02f * (V_dend[i] + 8.5f) / (expf((V_dend[i] + 8.5f) / 5) - 1);
+    xr = V_dend[i] + 8.5f;   /* 0/0 at -8.5 exactly: the limit 0.02 * 5 */
+    beta_r = xr == 0 ? 0.1f : 0.02f * xr / (expf(xr / 5) - 1);
     r_inf = alpha_r / (alpha_r + beta_r);
     tau_r = 5 / (alpha_r + beta_r);
     dr_dt = (r_inf - s[CAL_R][i]) / tau_r;
@@ -370,7 +380,8 @@ static void cell32(const p32 *P, float **s, int64_t i, float I_c, float iApp)
     s[POT_P][i] = P->DELTA * dp_dt + s[POT_P][i];
 
     /* Soma potassium X OMP:627-632 (literal 0.05f) */
-    alpha_x_s = 0.13f * (V_soma[i] + 25) / (1 - expf(-(V_soma[i] + 25) / 10));
+    xs = V_soma[i] + 25;     /* 0/0 at -25 exactly: the limit 0.13 * 10 */
+    alpha_x_s = xs == 0 ? 1.3f : 0.13f * xs / (1 - expf(-xs / 10));
     beta_x_s = 1.69f * expf(-0.0125f * (V_soma[i] + 35));
     x_inf_s = alpha_x_s / (alpha_x_s + beta_x_s);
     tau_x_s = 1 / (alpha_x_s + beta_x_s);
@@ -396,7 +407,8 @@ static void cell32(const p32 *P, float **s, int64_t i, float I_c, float iApp)
     s[NA_HA][i] = s[NA_HA][i] + P->DELTA * dh_dt_a;
 
     /* Axon potassium OMP:656-661 */
-    alpha_x_a = 0.13f * (V_axon[i] + 25) / (1 - expf(-(V_axon[i] + 25) / 10));
+    xa = V_axon[i] + 25;
+    alpha_x_a = xa == 0 ? 1.3f : 0.13f * xa / (1 - expf(-xa / 10));
     beta_x_a = 1.69f * expf(-0.0125f * (V_axon[i] + 35));
     x_inf_a = alpha_x_a / (alpha_x_a + beta_x_a);
     tau_x_a = 1 / (alpha_x_a + beta_x_a);

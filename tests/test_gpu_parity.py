@@ -528,13 +528,49 @@ def test_nonfinite_watchdog():
     say("watchdog")
     net = make_net(4, "fp64")
     v = np.full(4, -60.0)
-    v[2] = -8.5   # 0/0 in beta_r, as in the reference
+    v[2] = np.nan
     net.set_state("V_dend", v)
     with pytest.raises(I.InfoliError) as e:
         net.advance(1)
     assert e.value.code == -5
     assert np.isnan(net.get_state("Calcium_r")[2])
     assert np.isfinite(net.get_state("Calcium_r")[[0, 1, 3]]).all()
+    net.close()
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-9), ("fp32", 2e-4), ("fp32k", 2e-4)])
+def test_singular_voltages_take_the_limit_vs_oracle(prec, tol):
+    """Cells started exactly at V_dend = -8.5 mV or V_soma / V_axon = -25 mV
+    (0/0 in beta_r and alpha_x, NaN in the reference): the rates take their
+    limits 0.1 and 1.3, the network stays finite and matches the oracle,
+    which does the same, over 300 steps on a coupled random graph."""
+    say(f"singular voltages {prec}")
+    n, steps = 300, 300
+    rp, col, w, gcal, iapp = _random_case(n, p=0.05, seed=17)
+    st0 = O.initial_state(n)
+    st0[O.V_DEND, 0:100:3] = -8.5
+    st0[O.V_SOMA, 1:100:3] = -25.0
+    st0[O.V_AXON, 2:100:3] = -25.0
+    st0[O.V_SOMA, 150] = st0[O.V_AXON, 150] = -25.0
+    st0[O.V_DEND, 150] = -8.5
+    st0[O.G_CAL] = gcal
+    net = make_net(n, prec, rp, col, w, gcal, iapp)
+    for v in (O.V_DEND, O.V_SOMA, O.V_AXON):
+        net.set_state(v, st0[v])
+    for fm in ((0, 1) if prec != "fp64" else (0,)):
+        if fm:
+            net.close()
+            net = make_net(n, prec, rp, col, w, gcal, iapp)
+            for v in (O.V_DEND, O.V_SOMA, O.V_AXON):
+                net.set_state(v, st0[v])
+            net.set_exec(gather=I.GATHER_TILED, fast_math=1)
+        out = net.advance(steps, trace_stride=3)
+        assert np.isfinite(out).all()
+        if not fm:
+            O.set_fp64_degree(I.FP64_DEGREE_DEFAULT if prec == "fp32k" else 0)
+            st = st0.copy()
+            ref, _ = O.run(PRECS[prec], st, steps, rp, col, w, iapp_base=iapp, stride=3)
+            assert np.abs(out - ref).max() < tol
     net.close()
 
 

@@ -210,13 +210,28 @@ def test_synchrony_invariant():
         assert (out == ref).all(), prec
 
 
-def test_removable_singularity_gives_nan_like_reference():
-    # beta_r = 0.02 (V + 8.5) / (exp((V + 8.5)/5) - 1) is 0/0 at V = -8.5
-    # exactly; the reference has no guard, so the state becomes NaN.
-    st = O.initial_state(1)
-    st[O.V_DEND] = -8.5
-    O.run(O.FP64, st, 1)
-    assert np.isnan(st[O.CALCIUM_R, 0])
+@pytest.mark.parametrize("prec,delta,tol", [(O.FP64, 1e-4, 1e-8), (O.FP32, 1e-2, 2e-6), (O.FP32K, 1e-2, 2e-6)])
+@pytest.mark.parametrize("var,v0,gate,gate_value", [(O.V_DEND, -8.5, O.CALCIUM_R, 1.0),
+                                                    (O.V_SOMA, -25.0, O.POTASSIUM_X_S, 0.0),
+                                                    (O.V_AXON, -25.0, O.POTASSIUM_X_A, 0.0)])
+def test_removable_singularity_takes_its_limit(prec, delta, tol, var, v0, gate, gate_value):
+    """beta_r = 0.02 x / (exp(x/5) - 1), x = V_dend + 8.5, and alpha_x =
+    0.13 x / (1 - exp(-x/10)), x = V + 25, are 0/0 at x = 0 exactly (the
+    reference computes NaN there and the network follows).  The oracle, like
+    the GPU path, uses the limits 0.02 * 5 and 0.13 * 10: one step from the
+    singular voltage gives the midpoint of the steps from V0 -/+ delta
+    (the gate's update is smooth in V).  The gate value is chosen so the
+    update is sensitive to the rate (dr/dbeta_r = -0.01 r, dx/dalpha_x =
+    0.05 (1 - x)): a limit off by 10% moves the result by 1e-4 or more."""
+    out = []
+    for v in (v0 - delta, v0, v0 + delta):
+        st = O.initial_state(1)
+        st[var] = v
+        st[gate] = gate_value
+        O.run(prec, st, 1, trace=False)
+        assert np.isfinite(st).all()
+        out.append(st[gate, 0])
+    assert abs(out[1] - 0.5 * (out[0] + out[2])) < tol, out
 
 
 @pytest.mark.skipif(not os.path.exists(os.path.join(O.REF_DIR, "checker.x")),
