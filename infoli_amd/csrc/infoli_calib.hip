@@ -20,6 +20,22 @@
  *                                    workgroup b reads what workgroup b + s
  *                                    wrote, on the same XCD (s = 0, 8) or on
  *                                    another (s = 1, 3, 4)
+ *   infoli_calib.x sync              hand-over of partial sums inside one kernel,
+ *                                    in the sparse 100k step's shape: 245
+ *                                    workgroups of 1024 threads in 49 groups
+ *                                    of 5, each writing 2048 (sum,
+ *                                    compensation) pairs and 32 KB of other
+ *                                    state; the last of a group to arrive
+ *                                    merges the group's pairs.  Against the
+ *                                    same work as two kernels per step, with
+ *                                    (1) one agent-scope release / acquire per
+ *                                    workgroup, (2) device-coherent atomic
+ *                                    loads and stores of the pairs and no
+ *                                    fence, (3) a release per wave; counts
+ *                                    merged pairs that were not the current
+ *                                    step's.  Then a grid barrier of 256
+ *                                    workgroups (one per CU) with the same
+ *                                    three publication schemes.
  *
  * Every index is drawn in range on the host; grids are sized to the buffers.
  */
@@ -142,6 +158,139 @@ __global__ __launch_bounds__(256) void handover_read(const float *p, float *q, i
 {
     const size_t src = (size_t)((blockIdx.x + shift) % nb) * 1024, dst = (size_t)blockIdx.x * 1024;
     for (int i = threadIdx.x; i < 1024; i += 256) q[dst + i] = p[src + i] + 1.0f;
+}
+
+/* ---- in-kernel hand-over (sync mode) ---------------------------------- */
+
+constexpr int kSyncR = 2048;        // pairs per workgroup (receivers of a tile)
+constexpr int kSyncState = 8192;    // other floats each workgroup writes
+
+__global__ void sync_bump(int *it) { *it += 1; }
+
+/* one (sum, compensation) pair as a 64-bit word */
+__device__ __forceinline__ unsigned long long pack2(float a, float b)
+{
+    return (unsigned long long)__float_as_uint(a) | ((unsigned long long)__float_as_uint(b) << 32);
+}
+
+template <int V>
+__device__ __forceinline__ void sync_store(float2 *p, size_t i, float2 v)
+{
+    if constexpr (V == 2)
+        __hip_atomic_store(reinterpret_cast<unsigned long long *>(p) + i, pack2(v.x, v.y), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    else
+        p[i] = v;
+}
+
+template <int V>
+__device__ __forceinline__ float2 sync_load(const float2 *p, size_t i)
+{
+    if constexpr (V == 2) {
+        const unsigned long long u = __hip_atomic_load(reinterpret_cast<const unsigned long long *>(p) + i,
+                                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return make_float2(__uint_as_float((unsigned)u), __uint_as_float((unsigned)(u >> 32)));
+    }
+    return p[i];
+}
+
+/* the consumer's work for group g: merge the group's pairs, check them */
+template <int V>
+__device__ __forceinline__ void sync_merge(const float2 *part, float *out, int g, int group, int it, unsigned *err)
+{
+    unsigned bad = 0;
+    for (int r = threadIdx.x; r < kSyncR; r += blockDim.x) {
+        float acc = 0.f;
+        for (int k = 0; k < group; k++) {
+            const int wg = g * group + k;
+            const float2 v = sync_load<V>(part, (size_t)wg * kSyncR + r);
+            bad += (v.x != (float)(wg * kSyncR + r) || v.y != (float)it) ? 1u : 0u;
+            acc += v.x + v.y;
+        }
+        out[(size_t)g * kSyncR + r] = acc;
+    }
+    if (bad) atomicAdd(err, bad);
+}
+
+/* V = 0: producer only (the consumer is the next kernel); 1: agent-scope
+ * release / acquire once per workgroup; 2: device-coherent atomics, no
+ * fence; 3: an agent-scope release in every wave */
+template <int V>
+__global__ __launch_bounds__(1024) void sync_produce(float2 *part, float *state, float *out, unsigned *cnt,
+                                                     const int *it_p, int group, unsigned *err)
+{
+    const int wg = blockIdx.x, g = wg / group, it = *it_p;
+    for (int r = threadIdx.x; r < kSyncR; r += 1024)
+        sync_store<V>(part, (size_t)wg * kSyncR + r, make_float2((float)(wg * kSyncR + r), (float)it));
+    for (int i = threadIdx.x; i < kSyncState; i += 1024) state[(size_t)wg * kSyncState + i] += 1.0f;
+    if constexpr (V == 0) return;
+    __shared__ int s_last;
+    if constexpr (V == 3) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned old;
+        if constexpr (V == 1)
+            old = __hip_atomic_fetch_add(&cnt[g], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        else
+            old = __hip_atomic_fetch_add(&cnt[g], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int last = old == (unsigned)(group - 1);
+        if (last && V != 2) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (last) __hip_atomic_store(&cnt[g], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = last;
+    }
+    __syncthreads();
+    if (s_last) sync_merge<V>(part, out, g, group, it, err);
+}
+
+__global__ __launch_bounds__(1024) void sync_consume(const float2 *part, float *out, const int *it_p, int group,
+                                                     unsigned *err)
+{
+    sync_merge<0>(part, out, blockIdx.x, group, *it_p, err);
+}
+
+/* grid barrier over gridDim.x co-resident workgroups: every workgroup
+ * publishes a 4 KB slice, waits for all, then reads its neighbour's; V as
+ * above.  The wait is bounded (a timeout is counted and the loop goes on) */
+template <int V>
+__global__ __launch_bounds__(1024) void sync_barrier(float *slab, unsigned *cnt, int iters, unsigned *err)
+{
+    __shared__ int s_ok;
+    const int wg = blockIdx.x, nb = gridDim.x;
+    unsigned bad = 0;
+    for (int k = 0; k < iters; k++) {
+        float *mine = slab + (size_t)(k & 1) * nb * 1024 + (size_t)wg * 1024;
+        const float val = (float)(k * nb + wg);
+        if constexpr (V == 2) {
+            __hip_atomic_store(mine + threadIdx.x, val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            mine[threadIdx.x] = val;
+        }
+        if constexpr (V == 3) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const unsigned target = (unsigned)(k + 1) * nb;
+            if constexpr (V == 1)
+                __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            else
+                __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            unsigned spins = 0;
+            int ok = 1;
+            while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                if (++spins > (1u << 22)) { ok = 0; break; }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            if (V != 2) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            s_ok = ok;
+        }
+        __syncthreads();
+        if (!s_ok && threadIdx.x == 0) bad += 1u << 16;
+        const float *theirs = slab + (size_t)(k & 1) * nb * 1024 + (size_t)((wg + 1) % nb) * 1024;
+        float v;
+        if constexpr (V == 2) v = __hip_atomic_load(theirs + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else v = theirs[threadIdx.x];
+        bad += v != (float)(k * nb + (wg + 1) % nb) ? 1u : 0u;
+    }
+    if (bad) atomicAdd(err, bad);
 }
 
 static float time_ms(hipEvent_t e0, hipEvent_t e1)
@@ -326,8 +475,99 @@ int main(int argc, char **argv)
         CHK(hipStreamDestroy(s));
         CHK(hipFree(part));
         CHK(hipFree(st));
+    } else if (!strcmp(what, "sync")) {
+        const int group = 5, G = 49 * group, K = 100;
+        float2 *part;
+        float *state, *out, *slab;
+        unsigned *cnt, *err;
+        int *it;
+        CHK(hipMalloc(&part, (size_t)G * kSyncR * sizeof(float2)));
+        CHK(hipMalloc(&state, (size_t)G * kSyncState * sizeof(float)));
+        CHK(hipMalloc(&out, (size_t)(G / group) * kSyncR * sizeof(float)));
+        CHK(hipMalloc(&cnt, 64 * sizeof(unsigned)));
+        CHK(hipMalloc(&err, sizeof(unsigned)));
+        CHK(hipMalloc(&it, sizeof(int)));
+        hipStream_t s;
+        CHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        CHK(hipMemsetAsync(part, 0, (size_t)G * kSyncR * sizeof(float2), s));
+        CHK(hipMemsetAsync(state, 0, (size_t)G * kSyncState * sizeof(float), s));
+        CHK(hipMemsetAsync(cnt, 0, 64 * sizeof(unsigned), s));
+        CHK(hipMemsetAsync(it, 0, sizeof(int), s));
+        const char *names[] = {"two kernels (produce, then merge)", "one kernel, release/acquire per workgroup",
+                               "one kernel, device-coherent atomics, no fence", "one kernel, release per wave"};
+        for (int v = 0; v < 4; v++) {
+            CHK(hipMemsetAsync(err, 0, sizeof(unsigned), s));
+            CHK(hipStreamSynchronize(s));
+            hipGraph_t g;
+            hipGraphExec_t ge;
+            CHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            for (int k = 0; k < K; k++) {
+                hipLaunchKernelGGL(sync_bump, dim3(1), dim3(1), 0, s, it);
+                if (v == 0) {
+                    hipLaunchKernelGGL(sync_produce<0>, dim3(G), dim3(1024), 0, s, part, state, out, cnt,
+                                       (const int *)it, group, err);
+                    hipLaunchKernelGGL(sync_consume, dim3(G / group), dim3(1024), 0, s, (const float2 *)part, out,
+                                       (const int *)it, group, err);
+                } else if (v == 1) {
+                    hipLaunchKernelGGL(sync_produce<1>, dim3(G), dim3(1024), 0, s, part, state, out, cnt,
+                                       (const int *)it, group, err);
+                } else if (v == 2) {
+                    hipLaunchKernelGGL(sync_produce<2>, dim3(G), dim3(1024), 0, s, part, state, out, cnt,
+                                       (const int *)it, group, err);
+                } else {
+                    hipLaunchKernelGGL(sync_produce<3>, dim3(G), dim3(1024), 0, s, part, state, out, cnt,
+                                       (const int *)it, group, err);
+                }
+            }
+            CHK(hipStreamEndCapture(s, &g));
+            CHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+            float best = 1e30f;
+            for (int rep = 0; rep < 4; rep++) {
+                CHK(hipEventRecord(e0, s));
+                CHK(hipGraphLaunch(ge, s));
+                CHK(hipEventRecord(e1, s));
+                const float ms = time_ms(e0, e1);
+                best = ms < best ? ms : best;
+            }
+            unsigned nerr = 0;
+            CHK(hipMemcpy(&nerr, err, sizeof nerr, hipMemcpyDeviceToHost));
+            printf("hand-over, %-46s %.2f us per step, %u stale pairs of %lld merged\n", names[v],
+                   best * 1000 / K, nerr, (long long)4 * K * G * kSyncR);
+            CHK(hipGraphExecDestroy(ge));
+            CHK(hipGraphDestroy(g));
+        }
+        // grid barrier, one 1024-thread workgroup per CU (all co-resident)
+        int ncu = 0;
+        CHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, 0));
+        const int iters = 200;
+        CHK(hipMalloc(&slab, (size_t)2 * ncu * 1024 * sizeof(float)));
+        const char *bnames[] = {"", "release/acquire per workgroup", "device-coherent atomics, no fence",
+                                "release per wave"};
+        for (int v = 1; v < 4; v++) {
+            CHK(hipMemsetAsync(cnt, 0, sizeof(unsigned), s));
+            CHK(hipMemsetAsync(err, 0, sizeof(unsigned), s));
+            CHK(hipStreamSynchronize(s));
+            CHK(hipEventRecord(e0, s));
+            if (v == 1) hipLaunchKernelGGL(sync_barrier<1>, dim3(ncu), dim3(1024), 0, s, slab, cnt, iters, err);
+            if (v == 2) hipLaunchKernelGGL(sync_barrier<2>, dim3(ncu), dim3(1024), 0, s, slab, cnt, iters, err);
+            if (v == 3) hipLaunchKernelGGL(sync_barrier<3>, dim3(ncu), dim3(1024), 0, s, slab, cnt, iters, err);
+            CHK(hipEventRecord(e1, s));
+            const float ms = time_ms(e0, e1);
+            unsigned nerr = 0;
+            CHK(hipMemcpy(&nerr, err, sizeof nerr, hipMemcpyDeviceToHost));
+            printf("grid barrier of %d workgroups, %-34s %.2f us per barrier, %u stale reads, %u timeouts\n", ncu,
+                   bnames[v], ms * 1000 / iters, nerr & 0xFFFF, nerr >> 16);
+        }
+        CHK(hipFree(slab));
+        CHK(hipStreamDestroy(s));
+        CHK(hipFree(part));
+        CHK(hipFree(state));
+        CHK(hipFree(out));
+        CHK(hipFree(cnt));
+        CHK(hipFree(err));
+        CHK(hipFree(it));
     } else {
-        fprintf(stderr, "usage: infoli_calib.x copy [MiB] | gather [table] [xcd] | exp | launch\n");
+        fprintf(stderr, "usage: infoli_calib.x copy [MiB] | gather [table] [xcd] | exp | launch | sync\n");
         return 2;
     }
     return 0;

@@ -112,8 +112,10 @@ struct infoli_ctx {
     size_t h_stage_bytes = 0;
 
     int64_t *d_step = nullptr;
-    GridBarrier *d_bar = nullptr;
-    int *d_flag = nullptr;
+    AdvanceStatus *d_status = nullptr;   // barrier + watchdog flag of an advance
+    GridBarrier *d_bar = nullptr;        // &d_status->bar
+    int *d_flag = nullptr;               // &d_status->nonfinite
+    AdvanceStatus *h_status = nullptr;   // pinned copy read after each advance
     int64_t step = 0;
 
     std::vector<int64_t> row_ptr;
@@ -493,7 +495,6 @@ int launch_persist(infoli_ctx *c, const void *args, int nsteps, int grid)
     return INFOLI_OK;
 }
 
-__global__ void k_set_step(int64_t *d_step, int64_t v) { *d_step = v; }
 
 /* ------------------------------------------------------------------------ */
 /* copies and fills, all on the context's stream                             */
@@ -1264,10 +1265,12 @@ extern "C" int infoli_create(infoli_t **out, const infoli_params *params, int64_
         hipMalloc(&c->d_va[0], plane) != hipSuccess || hipMalloc(&c->d_va[1], plane) != hipSuccess ||
         hipMalloc(&c->d_deg, sizeof(int32_t) * c->npad) != hipSuccess ||
         hipMalloc(&c->d_step, sizeof(int64_t)) != hipSuccess ||
-        hipMalloc(&c->d_bar, sizeof(GridBarrier)) != hipSuccess ||
-        hipMalloc(&c->d_flag, sizeof(int)) != hipSuccess)
+        hipMalloc(&c->d_status, sizeof(AdvanceStatus)) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_status, sizeof(AdvanceStatus), hipHostMallocDefault) != hipSuccess)
         return fail(infoli_fail(INFOLI_ERR_NOMEM, "infoli_create: device allocation of %lld cells failed",
                                 (long long)n_cells));
+    c->d_bar = &c->d_status->bar;
+    c->d_flag = &c->d_status->nonfinite;
     // zeroed on the context's stream, ahead of the initial-state uploads on
     // that stream (a null-stream hipMemset is not ordered before them, and a
     // voltage plane zeroed after its upload drives the network to NaN)
@@ -1309,12 +1312,13 @@ extern "C" void infoli_destroy(infoli_t *c)
         if (e) (void)hipEventDestroy(e);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     void *bufs[] = {c->d_state, c->d_vd[0], c->d_vd[1], c->d_vs[0], c->d_vs[1], c->d_va[0], c->d_va[1], c->d_deg, c->d_slice_ptr, c->d_col, c->d_w,
-                    c->d_iapp_base, c->d_mask, c->d_tab, c->d_trace, c->d_step, c->d_bar, c->d_flag};
+                    c->d_iapp_base, c->d_mask, c->d_tab, c->d_trace, c->d_step, c->d_status};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     free_staged(c);
     free_tiled(c);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
+    if (c->h_status) (void)hipHostFree(c->h_status);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1784,6 +1788,7 @@ int capture_graph(infoli_ctx *c, const StepArgs<T> &a, int mode, int G, int grid
     if (ee != hipSuccess) return infoli_fail(INFOLI_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ee));
     c->graph = g;
     HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+    HIPCHK(hipGraphUpload(c->gexec, c->stream));   // its first replay then runs at full speed
     c->gkey = key;
     return INFOLI_OK;
 }
@@ -1824,10 +1829,11 @@ int run_steps(infoli_ctx *c, StepArgs<T> a, int64_t t, int64_t count)
         }
         return INFOLI_OK;
     }
+    // the device step counter holds t here: k_begin set it at the start of
+    // the advance, each replay adds its steps, and so do the bumps after
+    // steps launched one by one
     const int G = c->exec.steps_per_launch;
     a.d_step = c->d_step;
-    hipLaunchKernelGGL(k_set_step, dim3(1), dim3(1), 0, c->stream, c->d_step, t);
-    HIPCHK(hipGetLastError());
     int64_t done = 0;
     if (mode == INFOLI_EXEC_GRAPH) {
         RCHK(prepare_graph(c, a, count));
@@ -1837,8 +1843,13 @@ int run_steps(infoli_ctx *c, StepArgs<T> a, int64_t t, int64_t count)
         }
         StepArgs<T> b = a;
         b.d_step = nullptr;
-        c->n_direct += count - done;
+        const int64_t rem = count - done;
+        c->n_direct += rem;
         for (; done < count; done++) RCHK(launch_step(c, &b, t + done));
+        if (rem > 0) {
+            hipLaunchKernelGGL(k_bump_step, dim3(1), dim3(1), 0, c->stream, c->d_step, rem);
+            HIPCHK(hipGetLastError());
+        }
         return INFOLI_OK;
     }
     // persistent
@@ -1955,7 +1966,7 @@ int advance_impl(infoli_ctx *c, int64_t n_steps, int64_t stride, double *trace)
     for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
     c->prof_ev.clear();
 
-    hipLaunchKernelGGL(k_reset_barrier, dim3(1), dim3(1), 0, c->stream, c->d_bar);
+    hipLaunchKernelGGL(k_begin, dim3(1), dim3(1), 0, c->stream, c->d_step, c->step, c->d_status);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     int64_t t = c->step, left = n_steps, rec = 0;
@@ -1980,16 +1991,14 @@ int advance_impl(infoli_ctx *c, int64_t n_steps, int64_t stride, double *trace)
         const int last_buf = (int)(((n_steps + per_piece * stride_eff - 1) / (per_piece * stride_eff) - 1) & 1);
         RCHK(drain_records<T>(c, last_buf, per_piece, pending, trace, &rec));
     }
-    hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, c->stream, c->d_flag);
     hipLaunchKernelGGL(k_check_finite<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
                        (const T *)c->d_vd[c->step & 1], (const T *)c->d_vs[c->step & 1],
                        (const T *)c->d_va[c->step & 1], n, c->d_flag);
     HIPCHK(hipGetLastError());
-    int flag = 0;
-    GridBarrier gb{0, 0};
-    HIPCHK(hipMemcpyAsync(&flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(&gb, c->d_bar, sizeof gb, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->h_status, c->d_status, sizeof(AdvanceStatus), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    const int flag = c->h_status->nonfinite;
+    const GridBarrier gb = c->h_status->bar;
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
