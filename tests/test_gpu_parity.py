@@ -558,7 +558,7 @@ def _spiking_cell_states():
     return out
 
 
-@pytest.mark.parametrize("prec,tol", [("fp64", 1e-9), ("fp32", 2e-4), ("fp32k", 2e-4)])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-9), ("fp32", 5e-4), ("fp32k", 5e-4)])
 def test_singular_voltages_take_the_limit_vs_oracle(prec, tol):
     """alpha_x is 0/0 at V = -25 mV and beta_r at V_dend = -8.5 mV (NaN in
     the reference).  On a coupled random graph: cells exactly at V_soma or
@@ -566,8 +566,9 @@ def test_singular_voltages_take_the_limit_vs_oracle(prec, tol):
     V_dend = -8.5 mV from rest, over 20 steps (from that state forward Euler
     diverges after about 30 steps in FP64 as well).  The rates take their
     limits 1.3 and 0.1, the network stays finite and matches the oracle,
-    which does the same; with the fast compartment arithmetic it stays
-    finite."""
+    which does the same (FP64 to 1e-9; FP32 to 5e-4, the tolerance of the
+    other FP32 comparisons with spikes); with the fast compartment
+    arithmetic it stays finite."""
     say(f"singular voltages {prec}")
     n = 300
     rp, col, w, gcal, iapp = _random_case(n, p=0.05, seed=17)
