@@ -228,6 +228,21 @@ int64_t infoli_step_count(const infoli_t *ctx);
  * src/single_threaded/infoli.c:544-556, would compute now). */
 int infoli_gap_currents(infoli_t *ctx, double *i_c);
 
+/* infoli_gap_currents without allocating anything: the currents pass
+ * through a static device scratch array of INFOLI_NOALLOC_SCRATCH_BYTES per
+ * GPU instead of a buffer hipMalloc'd per call, and are copied straight
+ * into i_c (no pinned staging), so the call makes no hipMalloc, hipFree or
+ * other allocating call.  Same values as infoli_gap_currents, bit for bit.
+ * No reference counterpart: it is for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Limit: n_cells * (4 or 8 bytes) <= 16 MiB, i.e. 4,194,304
+ * cells in FP32, 2,097,152 in FP64 (else INFOLI_ERR_ARG).  The neighbour
+ * lists of the current strategy must be built (every infoli_advance and
+ * infoli_gap_currents builds them), else INFOLI_ERR_STATE.  All contexts on
+ * one GPU share the scratch array: calls that overlap on one GPU race. */
+#define INFOLI_NOALLOC_SCRATCH_BYTES (16 << 20)
+int infoli_gap_currents_noalloc(infoli_t *ctx, double *i_c);
+
 /* ---- execution strategy and timing -------------------------------------- */
 
 #define INFOLI_EXEC_AUTO        0  /* autotuned / heuristic choice           */

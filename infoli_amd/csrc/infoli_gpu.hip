@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -23,6 +24,15 @@
 #include "infoli_kernels.h"
 
 using namespace infoli;
+
+/* Device scratch of infoli_gap_currents_noalloc: one call's gap currents
+ * (n values in the context's precision) on their way to the caller.  Each
+ * GPU the library runs on holds its own copy until the process ends; all
+ * contexts on a GPU share it, so calls that overlap on one GPU race.  Host
+ * code never uses this name as an address: infoli_create resolves the
+ * device address once per device (scratch_setup) and the entry point passes
+ * that pointer on. */
+static __device__ __align__(256) unsigned char scratch[INFOLI_NOALLOC_SCRATCH_BYTES];
 
 #define HIPCHK(x)                                                                          \
     do {                                                                                   \
@@ -37,6 +47,31 @@ using namespace infoli;
         int r_ = (x);                    \
         if (r_ != INFOLI_OK) return r_;  \
     } while (0)
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};   // the scratch array's address on each device
+
+/* The scratch array's device address on `device`, which must be current;
+ * resolved once per device. */
+int scratch_setup(int device)
+{
+    if (device < 0 || device >= kMaxDevices)
+        return infoli_fail(INFOLI_ERR_NODEVICE, "device %d: at most %d devices", device, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return INFOLI_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+        return infoli_fail(INFOLI_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d failed: %s", device,
+                           hipGetErrorString(e));
+    g_scratch_ptr[device] = p;
+    return INFOLI_OK;
+}
+
+}  // namespace
 
 namespace {
 
@@ -1235,6 +1270,7 @@ extern "C" int infoli_create(infoli_t **out, const infoli_params *params, int64_
         return infoli_fail(INFOLI_ERR_NODEVICE, "infoli_create: device %d is %s, built for gfx950",
                            device, prop.gcnArchName);
     HIPCHK(hipSetDevice(device));
+    RCHK(scratch_setup(device));
 
     infoli_ctx *c = new (std::nothrow) infoli_ctx();
     if (!c) return infoli_fail(INFOLI_ERR_NOMEM, "infoli_create: no host memory");
@@ -2041,6 +2077,32 @@ extern "C" int infoli_prepare(infoli_t *c, int64_t n_steps, int64_t trace_stride
     return is_f32(c) ? prepare_impl<float>(c, n_steps, trace_stride) : prepare_impl<double>(c, n_steps, trace_stride);
 }
 
+namespace {
+
+/* the gap-current kernel of the context's layout, writing n values of the
+ * context's precision to device buffer `out` */
+int launch_gap_currents(infoli_ctx *c, void *out)
+{
+    const int64_t nblk = c->npad / kBlock;
+    const void *fn = hyb_active(c) ? k_gap_hyb_tab[lidx(c->exec.lanes_per_cell)]
+                                   : k_gap_tab[c->prec][lidx(c->exec.lanes_per_cell)];
+    int64_t t = c->step;
+    hipError_t e;
+    if (is_f32(c)) {
+        StepArgs<float> a = make_args<float>(c, 0);
+        void *kargs[] = {&a, &t, &out};
+        e = hipLaunchKernel(fn, dim3((unsigned)nblk), dim3(kBlock), kargs, 0, c->stream);
+    } else {
+        StepArgs<double> a = make_args<double>(c, 0);
+        void *kargs[] = {&a, &t, &out};
+        e = hipLaunchKernel(fn, dim3((unsigned)nblk), dim3(kBlock), kargs, 0, c->stream);
+    }
+    if (e != hipSuccess) return infoli_fail(INFOLI_ERR_HIP, "gap-current kernel launch failed: %s", hipGetErrorString(e));
+    return INFOLI_OK;
+}
+
+}  // namespace
+
 extern "C" int infoli_gap_currents(infoli_t *c, double *i_c)
 {
     infoli_clear_error();
@@ -2048,27 +2110,51 @@ extern "C" int infoli_gap_currents(infoli_t *c, double *i_c)
     if (!i_c) return infoli_fail(INFOLI_ERR_ARG, "infoli_gap_currents: null output");
     HIPCHK(hipSetDevice(c->device));
     RCHK(ensure_layout(c));
-    const int64_t nblk = c->npad / kBlock;
     void *out = nullptr;
     HIPCHK(hipMalloc(&out, (size_t)c->n * c->tsize));
-    const void *fn = hyb_active(c) ? k_gap_hyb_tab[lidx(c->exec.lanes_per_cell)]
-                                   : k_gap_tab[c->prec][lidx(c->exec.lanes_per_cell)];
-    int64_t t = c->step;
-    int rc = INFOLI_OK;
-    if (is_f32(c)) {
-        StepArgs<float> a = make_args<float>(c, 0);
-        void *kargs[] = {&a, &t, &out};
-        if (hipLaunchKernel(fn, dim3((unsigned)nblk), dim3(kBlock), kargs, 0, c->stream) != hipSuccess)
-            rc = infoli_fail(INFOLI_ERR_HIP, "gap-current kernel launch failed");
-    } else {
-        StepArgs<double> a = make_args<double>(c, 0);
-        void *kargs[] = {&a, &t, &out};
-        if (hipLaunchKernel(fn, dim3((unsigned)nblk), dim3(kBlock), kargs, 0, c->stream) != hipSuccess)
-            rc = infoli_fail(INFOLI_ERR_HIP, "gap-current kernel launch failed");
-    }
+    int rc = launch_gap_currents(c, out);
     if (rc == INFOLI_OK) rc = download_values(c, out, i_c, c->n);
     (void)hipFree(out);
     return rc;
+}
+
+extern "C" int infoli_gap_currents_noalloc(infoli_t *c, double *i_c)
+{
+    infoli_clear_error();
+    RCHK(validate_ctx(c, "infoli_gap_currents_noalloc"));
+    if (!i_c) return infoli_fail(INFOLI_ERR_ARG, "infoli_gap_currents_noalloc: null output");
+    const size_t bytes = (size_t)c->n * c->tsize;
+    if (bytes > (size_t)INFOLI_NOALLOC_SCRATCH_BYTES)
+        return infoli_fail(INFOLI_ERR_ARG,
+                           "infoli_gap_currents_noalloc: %lld cells need %zu bytes of scratch, the static "
+                           "buffer holds %zu (%zu cells in this precision); use infoli_gap_currents",
+                           (long long)c->n, bytes, (size_t)INFOLI_NOALLOC_SCRATCH_BYTES,
+                           (size_t)INFOLI_NOALLOC_SCRATCH_BYTES / c->tsize);
+    void *out = c->device < kMaxDevices ? g_scratch_ptr[c->device] : nullptr;
+    if (!out) return infoli_fail(INFOLI_ERR_STATE, "infoli_gap_currents_noalloc: no scratch on device %d", c->device);
+    // the sliced-ELL lists the kernel reads must already be on the device:
+    // building them would allocate
+    if (c->sell_L != c->exec.lanes_per_cell)
+        return infoli_fail(INFOLI_ERR_STATE, "infoli_gap_currents_noalloc: the neighbour lists of the current "
+                                             "strategy are not built (infoli_advance or infoli_gap_currents "
+                                             "builds them)");
+    HIPCHK(hipSetDevice(c->device));
+    RCHK(launch_gap_currents(c, out));
+    // straight into the caller's array: no staging buffer
+    HIPCHK(hipMemcpyAsync(i_c, out, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (is_f32(c)) {
+        // widen in place from the end: value i's double overwrites floats
+        // 2i and 2i + 1, both already widened
+        unsigned char *b = reinterpret_cast<unsigned char *>(i_c);
+        for (int64_t i = c->n - 1; i >= 0; i--) {
+            float v;
+            memcpy(&v, b + (size_t)i * sizeof(float), sizeof v);
+            const double d = (double)v;
+            memcpy(b + (size_t)i * sizeof(double), &d, sizeof d);
+        }
+    }
+    return INFOLI_OK;
 }
 
 extern "C" int infoli_last_timing(const infoli_t *c, double *advance_ms, double *kernel_ms,

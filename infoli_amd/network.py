@@ -92,6 +92,7 @@ def library():
         "infoli_last_launches": ([P, P, P, P], i32),
         "infoli_step_count": ([P], i64),
         "infoli_gap_currents": ([P, P], i32),
+        "infoli_gap_currents_noalloc": ([P, P], i32),
         "infoli_get_exec": ([P, P], i32),
         "infoli_set_exec": ([P, P], i32),
         "infoli_autotune": ([P, i64, P], i32),
@@ -328,9 +329,15 @@ class Network:
                "last_launches")
         return dict(graph_replays=r.value, direct_steps=d.value, captures=c.value)
 
-    def gap_currents(self):
+    def gap_currents(self, noalloc=False):
+        """I_c of every cell from the current V_dend.  noalloc: through the
+        entry point that allocates nothing (static device scratch, at most
+        16 MiB of values)."""
         out = np.zeros(self.n, np.float64)
-        _check(library().infoli_gap_currents(self._h, _p(out)), "gap_currents")
+        if noalloc:
+            _check(library().infoli_gap_currents_noalloc(self._h, _p(out)), "gap_currents_noalloc")
+        else:
+            _check(library().infoli_gap_currents(self._h, _p(out)), "gap_currents")
         return out
 
     def synchronize(self):
