@@ -47,9 +47,19 @@ extern "C" {
 /* ---- arithmetic of the step ------------------------------------------ */
 /* FP64: the double-precision arithmetic of src/single_threaded/infoli.c
  *       (mod_prec = double, src/single_threaded/infoli.h:68).
- * FP32: the float arithmetic of src/openmp/infoli.c:496-679
- *       (mod_prec = float, src/openmp/infoli.h:39-40), plain neighbour sums.
- * FP32_KAHAN: FP32 with each cell's neighbour sum compensated (TwoSum). */
+ * FP32: float arithmetic (mod_prec = float, src/openmp/infoli.h:39-40),
+ *       plain neighbour sums.  The compartments follow
+ *       src/openmp/infoli.c:548-679 (libm-accurate expf, IEEE division, the
+ *       reference's literals and association).  The gap-junction term of
+ *       :512-518 is the same formula evaluated with the hardware exp2,
+ *       exp(-V^2/100) = exp2(V^2 * (-log2(e)/100)), and one fused
+ *       multiply-add for 0.8 e + 0.2, so traces differ from an expf
+ *       evaluation by up to 2e-4 mV over 2,000 steps without spikes and
+ *       5e-4 mV with them.
+ * FP32_KAHAN: FP32 with each cell's neighbour sum compensated (TwoSum).
+ * In every arithmetic the removable singularities of beta_r (V_dend = -8.5
+ * mV, src/openmp/infoli.c:568) and alpha_x (V = -25 mV, :627, :656) take
+ * their limits 0.1 and 1.3 where the reference computes 0/0 = NaN. */
 #define INFOLI_FP64        0
 #define INFOLI_FP32        1
 #define INFOLI_FP32_KAHAN  2
