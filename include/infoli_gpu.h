@@ -387,7 +387,8 @@ int infoli_read_iapp_file(const char *path, int64_t n, int64_t max_steps,
  * checks them against the network: every sender id inside its tile's staged
  * chunk (padding slots too, since they are read), each row's junctions in
  * row order and each junction in exactly one tile, sender ranges ascending
- * per block, slice widths, receiver lists that are permutations.  Then one
+ * per block, slice widths, receiver lists that are permutations ordered by
+ * in-chunk degree, descending.  Then one
  * step's scratch traffic as the kernels index it: every partial-sum and
  * dendrite-sum slot the compartment kernel reads is written by the launch
  * before it, and with ode_split 2 the `nsa` soma/axon workgroups update every

@@ -2347,6 +2347,9 @@ int check_tiled(const CsrView &g, size_t tsize, size_t psz, int gather, int L, i
                 const int r = tb.rlist[td.rl0 + k];
                 if (r >= R || seen[r]) return infoli_fail(INFOLI_ERR_STATE, "layout: tile %d's receiver list is not a permutation", ti);
                 seen[r] = 1;
+                // the kernels take a slice's fewest valid slots from its last row
+                if (k > 0 && tb.rdeg[td.rl0 + k] > tb.rdeg[td.rl0 + k - 1])
+                    return infoli_fail(INFOLI_ERR_STATE, "layout: tile %d's receivers are not by degree, descending", ti);
             }
             for (int sl = 0; sl < td.nsl; sl++) {
                 const int64_t base = tb.tsp[td.sl0 + sl], span = tb.tsp[td.sl0 + sl + 1] - base;

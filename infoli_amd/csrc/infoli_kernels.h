@@ -989,35 +989,35 @@ template <typename T> struct Vec16;
 template <> struct Vec16<float> { using type = float4; };
 template <> struct Vec16<double> { using type = double2; };
 
-/* V_dend[s0 .. s0+cn) into LDS: 16-byte loads, four per thread in flight
- * before their LDS stores (the source is 16-byte aligned: chunks start at
- * multiples of 64 cells of a 256-byte aligned array).  Loads and stores
- * take an always valid index, so nothing branches: a lane past the end
- * rewrites one of the last 64 vectors with its own value (64 distinct
- * addresses per wavefront, so the rewrites do not pile onto one bank). */
+/* V_dend[s0 .. s0+cn) into LDS with the direct global-to-LDS loads of
+ * gfx950 (global_load_lds_dwordx4): wavefront w copies the 1 KiB blocks
+ * w, w + NW, ..., lane l's 16 bytes landing at the block's LDS address
+ * (M0) + 16 l, so the data never passes through registers and the loads
+ * are issued back to back.  Lanes past the last whole vector stay off; the
+ * cn % 4 (FP32) or cn % 2 (FP64) remaining values are copied by ordinary
+ * loads and stores.  The source is 16-byte aligned (chunks start at
+ * multiples of 64 cells of a 256-byte aligned array).  Returns once the
+ * copy has landed in LDS (the caller's barrier then publishes it). */
 template <typename T, int NT>
 __device__ __forceinline__ void tile_chunk_to_lds(const T *__restrict__ src, T *__restrict__ dst, int64_t cn,
                                                   int tid)
 {
-    using V = typename Vec16<T>::type;
-    constexpr int E = 16 / (int)sizeof(T), Q = 4;
+    constexpr int E = 16 / (int)sizeof(T), NW = NT / 64;
     const int nv = (int)(cn / E);
-    const V *__restrict__ sv = reinterpret_cast<const V *>(src);
-    V *__restrict__ dv = reinterpret_cast<V *>(dst);
-#pragma unroll 1
-    for (int b = 0; b < nv; b += Q * NT) {
-        V x[Q];
-        int j[Q];
-#pragma unroll
-        for (int q = 0; q < Q; q++) {
-            const int i = b + q * NT + tid;
-            j[q] = i < nv ? i : max(nv - 1 - (i & 63), 0);
-            x[q] = sv[j[q]];
-        }
-#pragma unroll
-        for (int q = 0; q < Q; q++) dv[j[q]] = x[q];
+    const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const char *sb = reinterpret_cast<const char *>(src);
+    char *db = reinterpret_cast<char *>(dst);
+    const int nb = (nv + 63) >> 6;   // blocks of 64 vectors
+#pragma unroll 4
+    for (int blk = wave; blk < nb; blk += NW) {
+        const int i = blk * 64 + lane;
+        if (i < nv)
+            __builtin_amdgcn_global_load_lds(sb + (size_t)i * 16,
+                                             (__attribute__((address_space(3))) void *)(db + (size_t)blk * 1024), 16, 0,
+                                             0);
     }
     for (int64_t i = (int64_t)nv * E + tid; i < cn; i += NT) dst[i] = src[i];
+    __builtin_amdgcn_s_waitcnt(0);   // every load, the LDS ones included, has completed
 }
 
 /* load from a wavefront-uniform pointer plus a per-lane 32-bit byte offset
@@ -1355,15 +1355,13 @@ __device__ __forceinline__ void tile_rows(const StepArgs<T> &a, const TileDesc t
         }
         if (sl < td.nsl) {
             Acc<T, KAHAN> acc;
-            // the fewest valid slots of any lane in the wave (all lanes are
-            // active here): below it no slot needs the degree check
+            // the fewest valid slots of any lane in the wave: below it no
+            // slot needs the degree check.  A tile lists its receivers by
+            // in-chunk degree, descending, so the slice's last row (lane 63
+            // holds its last position) has the smallest degree dmin, and the
+            // fewest valid slots are dmin / L
             int nvm = 0;
-            if constexpr (sizeof(T) == 4) {
-                int m = d > l ? (d - l + L - 1) / L : 0;
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1) m = min(m, __shfl_xor(m, off));
-                nvm = __builtin_amdgcn_readfirstlane(m);
-            }
+            if constexpr (sizeof(T) == 4) nvm = __builtin_amdgcn_readlane(d, 63) / L;
             if (d > 0) {
                 const T vi = vin[cellb + rloc];
                 const int64_t base0 = a.tsp[td.sl0 + sl];
