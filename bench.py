@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 # `infoli_amd/bin/infoli_calib.x exp`: the ceiling of the gap-junction terms,
 # one exp each, when the sums run from on-chip memory (the dense network)
 VEXP_PEAK_PER_S = 13.0e12
-GATHER_NAMES = {0: "direct", 1: "lds", 2: "staged", 3: "tiled", 4: "sweep"}
+GATHER_NAMES = {0: "direct", 1: "lds", 2: "staged", 3: "tiled", 4: "sweep", 5: "nets"}
 TILED_KERNELS = {0: "k_tile_gj+k_tile_ode", 1: "k_tile_gj+k_tile_ode3", 2: "k_tile_gj+k_tile_dend"}
 SWEEP_KERNELS = {0: "k_tile_sweep", 1: "k_tile_sweep+k_tile_ode3"}
 
@@ -42,7 +42,9 @@ def kernel_name(exec_cfg):
         return TILED_KERNELS[exec_cfg["ode_split"]]
     if exec_cfg["gather"] == 4:
         return SWEEP_KERNELS[exec_cfg["ode_split"]]
-    return "k_step"
+    if exec_cfg["gather"] == 5:
+        return "k_persist_nets"
+    return "k_persist" if exec_cfg["mode"] == 3 else "k_step"
 
 
 def measured_traffic(kernel, workload, prec):
@@ -224,7 +226,7 @@ def main():
     ap.add_argument("--stride", type=int, default=100)
     ap.add_argument("--mode", default="auto", choices=["auto", "graph", "persistent", "step", "tune"])
     ap.add_argument("--lanes", type=int, default=0)
-    ap.add_argument("--gather", default="auto", choices=["auto", "direct", "lds", "staged", "tiled", "sweep"])
+    ap.add_argument("--gather", default="auto", choices=["auto", "direct", "lds", "staged", "tiled", "sweep", "nets"])
     ap.add_argument("--ode-split", type=int, default=-1,
                     help="compartment placement with the tiled gathers (0, 1, 2; default: heuristic)")
     ap.add_argument("--lds-kb", type=int, default=0)
@@ -263,8 +265,9 @@ def main():
         spl = args.steps_per_launch or graph_steps(args.steps, 500 if wl.get("partition") is not None else 100)
         net.set_exec(steps_per_launch=spl)
         if args.gather != "auto" or args.ode_split >= 0:
-            g = {"direct": 0, "lds": 1, "staged": 2, "tiled": 3, "sweep": 4}.get(args.gather)
-            net.set_exec(gather=g, ode_split=args.ode_split if args.ode_split >= 0 else None)
+            g = {"direct": 0, "lds": 1, "staged": 2, "tiled": 3, "sweep": 4, "nets": 5}.get(args.gather)
+            net.set_exec(gather=g, ode_split=args.ode_split if args.ode_split >= 0 else None,
+                         mode=I.EXEC_PERSISTENT if g == 5 else None)
         if args.mode == "graph":
             net.set_exec(mode=I.EXEC_GRAPH)
         elif args.mode == "persistent":

@@ -272,6 +272,15 @@ int infoli_gap_currents_noalloc(infoli_t *ctx, double *i_c);
                                       per receiver block walking its chunks in
                                       order, the partial sums merged in LDS
                                       (same results as TILED at equal shapes)  */
+#define INFOLI_GATHER_NETS      5  /* whole networks per workgroup (the pieces
+                                      of a block-diagonal network that no
+                                      junction joins, up to 2048 cells each in
+                                      FP32 and 1024 in FP64): V_dend in LDS,
+                                      the state in registers, steps_per_launch
+                                      steps per launch with no barrier between
+                                      workgroups; persistent mode only, one
+                                      thread per cell (the results of the other
+                                      gathers at one lane per cell)            */
 
 typedef struct infoli_exec {
     int     mode;            /* INFOLI_EXEC_*                                 */

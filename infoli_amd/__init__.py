@@ -19,7 +19,11 @@ from .network import (  # noqa: F401
     GATHER_STAGED,
     GATHER_TILED,
     GATHER_SWEEP,
+    GATHER_NETS,
     VARS,
+    ERR_ARG,
+    ERR_NONFINITE,
+    ERR_STATE,
     InfoliError,
     Network,
     Params,

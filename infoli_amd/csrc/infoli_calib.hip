@@ -33,9 +33,11 @@
  *                                    loads and stores of the pairs and no
  *                                    fence, (3) a release per wave; counts
  *                                    merged pairs that were not the current
- *                                    step's.  Then a grid barrier of 256
- *                                    workgroups (one per CU) with the same
- *                                    three publication schemes.
+ *                                    step's.  Then a barrier of all the
+ *                                    workgroups (one per CU), and of groups of
+ *                                    16, 8, 5 and 2 workgroups that wait only
+ *                                    for each other, with the same three
+ *                                    publication schemes.
  *
  * Every index is drawn in range on the host; grids are sized to the buffers.
  */
@@ -252,14 +254,18 @@ __global__ __launch_bounds__(1024) void sync_consume(const float2 *part, float *
  * publishes a 4 KB slice, waits for all, then reads its neighbour's; V as
  * above.  The wait is bounded (a timeout is counted and the loop goes on) */
 template <int V>
-__global__ __launch_bounds__(1024) void sync_barrier(float *slab, unsigned *cnt, int iters, unsigned *err)
+__global__ __launch_bounds__(1024) void sync_barrier(float *slab, unsigned *cnt_all, int iters, unsigned *err, int group)
 {
     __shared__ int s_ok;
-    const int wg = blockIdx.x, nb = gridDim.x;
+    // workgroups [g * group, (g + 1) * group) synchronise among themselves on
+    // cnt_all[g * 16] (their own 64-byte line); group = gridDim.x is the grid barrier
+    const int wg = blockIdx.x, nb = group, g = blockIdx.x / group, wl = blockIdx.x % group;
+    unsigned *cnt = cnt_all + (size_t)g * 16;
+    slab += (size_t)g * 2 * group * 1024;
     unsigned bad = 0;
     for (int k = 0; k < iters; k++) {
-        float *mine = slab + (size_t)(k & 1) * nb * 1024 + (size_t)wg * 1024;
-        const float val = (float)(k * nb + wg);
+        float *mine = slab + (size_t)(k & 1) * nb * 1024 + (size_t)wl * 1024;
+        const float val = (float)(k * nb + wl + wg * 0);
         if constexpr (V == 2) {
             __hip_atomic_store(mine + threadIdx.x, val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
@@ -284,11 +290,11 @@ __global__ __launch_bounds__(1024) void sync_barrier(float *slab, unsigned *cnt,
         }
         __syncthreads();
         if (!s_ok && threadIdx.x == 0) bad += 1u << 16;
-        const float *theirs = slab + (size_t)(k & 1) * nb * 1024 + (size_t)((wg + 1) % nb) * 1024;
+        const float *theirs = slab + (size_t)(k & 1) * nb * 1024 + (size_t)((wl + 1) % nb) * 1024;
         float v;
         if constexpr (V == 2) v = __hip_atomic_load(theirs + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         else v = theirs[threadIdx.x];
-        bad += v != (float)(k * nb + (wg + 1) % nb) ? 1u : 0u;
+        bad += v != (float)(k * nb + (wl + 1) % nb) ? 1u : 0u;
     }
     if (bad) atomicAdd(err, bad);
 }
@@ -541,23 +547,31 @@ int main(int argc, char **argv)
         CHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, 0));
         const int iters = 200;
         CHK(hipMalloc(&slab, (size_t)2 * ncu * 1024 * sizeof(float)));
+        unsigned *gcnt;
+        CHK(hipMalloc(&gcnt, (size_t)ncu * 16 * sizeof(unsigned)));
         const char *bnames[] = {"", "release/acquire per workgroup", "device-coherent atomics, no fence",
                                 "release per wave"};
-        for (int v = 1; v < 4; v++) {
-            CHK(hipMemsetAsync(cnt, 0, sizeof(unsigned), s));
-            CHK(hipMemsetAsync(err, 0, sizeof(unsigned), s));
-            CHK(hipStreamSynchronize(s));
-            CHK(hipEventRecord(e0, s));
-            if (v == 1) hipLaunchKernelGGL(sync_barrier<1>, dim3(ncu), dim3(1024), 0, s, slab, cnt, iters, err);
-            if (v == 2) hipLaunchKernelGGL(sync_barrier<2>, dim3(ncu), dim3(1024), 0, s, slab, cnt, iters, err);
-            if (v == 3) hipLaunchKernelGGL(sync_barrier<3>, dim3(ncu), dim3(1024), 0, s, slab, cnt, iters, err);
-            CHK(hipEventRecord(e1, s));
-            const float ms = time_ms(e0, e1);
-            unsigned nerr = 0;
-            CHK(hipMemcpy(&nerr, err, sizeof nerr, hipMemcpyDeviceToHost));
-            printf("grid barrier of %d workgroups, %-34s %.2f us per barrier, %u stale reads, %u timeouts\n", ncu,
-                   bnames[v], ms * 1000 / iters, nerr & 0xFFFF, nerr >> 16);
+        // the grid barrier, then groups of workgroups that synchronise only
+        // among themselves (all groups at once, ncu workgroups in all)
+        for (int group : {ncu, 16, 8, 5, 2}) {
+            const int grid = ncu / group * group;
+            for (int v = 1; v < 4; v++) {
+                CHK(hipMemsetAsync(gcnt, 0, (size_t)ncu * 16 * sizeof(unsigned), s));
+                CHK(hipMemsetAsync(err, 0, sizeof(unsigned), s));
+                CHK(hipStreamSynchronize(s));
+                CHK(hipEventRecord(e0, s));
+                if (v == 1) hipLaunchKernelGGL(sync_barrier<1>, dim3(grid), dim3(1024), 0, s, slab, gcnt, iters, err, group);
+                if (v == 2) hipLaunchKernelGGL(sync_barrier<2>, dim3(grid), dim3(1024), 0, s, slab, gcnt, iters, err, group);
+                if (v == 3) hipLaunchKernelGGL(sync_barrier<3>, dim3(grid), dim3(1024), 0, s, slab, gcnt, iters, err, group);
+                CHK(hipEventRecord(e1, s));
+                const float ms = time_ms(e0, e1);
+                unsigned nerr = 0;
+                CHK(hipMemcpy(&nerr, err, sizeof nerr, hipMemcpyDeviceToHost));
+                printf("barrier of %3d workgroups (%3d groups), %-34s %.2f us per barrier, %u stale reads, %u timeouts\n",
+                       group, grid / group, bnames[v], ms * 1000 / iters, nerr & 0xFFFF, nerr >> 16);
+            }
         }
+        CHK(hipFree(gcnt));
         CHK(hipFree(slab));
         CHK(hipStreamDestroy(s));
         CHK(hipFree(part));

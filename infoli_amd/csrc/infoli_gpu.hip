@@ -134,6 +134,16 @@ struct infoli_ctx {
     void *d_tw = nullptr, *d_part = nullptr;
     void *d_dpre = nullptr;   // ode_split 2: [2][npad] dendrite current sum before I_c, and I_h
 
+    // whole networks per workgroup (GATHER_NETS): cell ranges [rng[g], rng[g+1])
+    bool nets_valid = false;
+    NetsRange *d_rng = nullptr;
+    unsigned char *d_nets_lists = nullptr;   // the ranges' neighbour lists kept in LDS
+    int64_t nets_nr = 0;      // ranges = workgroups
+    int nets_block = 0;       // threads per workgroup
+    int nets_cpt = 1;         // cells per thread
+    size_t nets_lds = 0;      // dynamic LDS per workgroup
+    int64_t nets_in_lds = 0;  // ranges whose lists are in LDS
+
     void *d_iapp_base = nullptr;
     uint8_t *d_mask = nullptr;
     double amp = 0;
@@ -161,6 +171,7 @@ struct infoli_ctx {
     std::vector<int64_t> part;
 
     infoli_exec exec{};
+    int user_mode = -1;   // launch mode the caller set with infoli_set_exec (-1: none)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     GraphKey gkey;
@@ -341,6 +352,12 @@ const void *const k_persist_tab[3][3][5] = {
     INFOLI_G_TABLE(k_persist, float, false),
     INFOLI_G_TABLE(k_persist, float, true),
 };
+/* [precision][cells per thread - 1] (FP64: one cell per thread) */
+const void *const k_persist_nets_tab[3][2] = {
+    {(const void *)k_persist_nets<double, false, 1>, (const void *)k_persist_nets<double, false, 1>},
+    {(const void *)k_persist_nets<float, false, 1>, (const void *)k_persist_nets<float, false, 2>},
+    {(const void *)k_persist_nets<float, true, 1>, (const void *)k_persist_nets<float, true, 2>},
+};
 /* [precision][threads: 256, 512, 1024][L] */
 #define INFOLI_NT_TABLE(T, K) \
     {INFOLI_L_TABLE(k_tile_gj, T, K, 256), INFOLI_L_TABLE(k_tile_gj, T, K, 512), \
@@ -506,8 +523,11 @@ int launch_step(infoli_ctx *c, const void *args, int64_t k)
     return INFOLI_OK;
 }
 
+size_t nets_lds_bytes(const infoli_ctx *c) { return c->nets_lds; }
+
 int persist_grid(infoli_ctx *c)
 {
+    if (c->exec.gather == GATHER_NETS) return (int)c->nets_nr;   // one workgroup per cell range
     const void *fn = k_persist_tab[c->prec][c->exec.gather][lidx(c->exec.lanes_per_cell)];
     int per_cu = 0;
     allow_dyn_lds(fn, dyn_lds_bytes(c));
@@ -520,6 +540,18 @@ int persist_grid(infoli_ctx *c)
 
 int launch_persist(infoli_ctx *c, const void *args, int nsteps, int grid)
 {
+    if (c->exec.gather == GATHER_NETS) {
+        // whole networks per workgroup: no grid barrier, nothing to reset
+        const void *fn = k_persist_nets_tab[c->prec][c->nets_cpt - 1];
+        const NetsRange *rng = c->d_rng;
+        const unsigned char *lists = c->d_nets_lists;
+        int ns = nsteps;
+        void *kargs[] = {const_cast<void *>(args), &rng, &lists, &ns};
+        allow_dyn_lds(fn, nets_lds_bytes(c));
+        HIPCHK(hipLaunchKernel(fn, dim3((unsigned)grid), dim3((unsigned)c->nets_block), kargs, nets_lds_bytes(c),
+                               c->stream));
+        return INFOLI_OK;
+    }
     const void *fn = k_persist_tab[c->prec][c->exec.gather][lidx(c->exec.lanes_per_cell)];
     GridBarrier *gb = c->d_bar;
     int ns = nsteps;
@@ -587,10 +619,14 @@ int upload_sell(infoli_ctx *c, int L)
     std::vector<T> w(std::max<int64_t>(ne, 1), T(0));
     for (int64_t i = 0; i < n; i++) {
         const int64_t s = i / CPW, cw = i % CPW, r0 = c->row_ptr[i];
-        for (int p = 0; p < deg[i]; p++) {
+        // padding slots (past the row's end, up to the slice width) name the
+        // row's own cell with conductance 0: read, never added, and inside
+        // the row's network (GATHER_NETS reads V_dend of that network only)
+        const int64_t W = (sp[s + 1] - sp[s]) / 64;
+        for (int p = 0; p < W * L; p++) {
             const int64_t e = sp[s] + (int64_t)(p / L) * 64 + cw * L + (p % L);
-            col[e] = c->col[r0 + p];
-            w[e] = (T)c->w[r0 + p];
+            col[e] = p < deg[i] ? c->col[r0 + p] : (int32_t)i;
+            if (p < deg[i]) w[e] = (T)c->w[r0 + p];
         }
     }
     (void)hipFree(c->d_col); c->d_col = nullptr;
@@ -942,6 +978,225 @@ int64_t soma_axon_workgroups(const infoli_ctx *c)
     return std::min(groups, std::max(free_slots, (groups + 3) / 4));
 }
 
+/* ---- whole networks per workgroup (GATHER_NETS) ---------------------- */
+
+/* work of a piece of the network in pair-term units: its junctions plus
+ * the compartment update of each cell, which costs about as much VALU work
+ * as 24 junction terms (22 exp and 34 divisions against 1 exp and 13
+ * arithmetic operations) */
+constexpr double kOdeTerms = 24.0;
+/* Step-time model of the choice between whole networks per workgroup and
+ * the graph-launched strategies, fitted to FP32 compensated runs on the
+ * MI355X (sweep_256: 256 networks of 256 cells, 7.6 us per step persistent
+ * against 9.9 us tiled; 560 networks of 300-700 cells, 31 against 27 us):
+ * a range's work costs 0.5 ns per term unit on its CU (one workgroup per
+ * CU, its latency barely hidden), the graph's whole work 0.27 ns per unit
+ * spread over the CUs plus 6.5 us per step for its two dependent kernels. */
+constexpr double kNetsUsPerTerm = 5.0e-4;
+constexpr double kGraphUsPerTerm = 2.7e-4;
+constexpr double kGraphStepUs = 6.5;
+
+/* The cuts b that no junction crosses (every row of a cell below b names
+ * only senders below b, and every row at or above b only senders at or
+ * above b), from the CSR alone: [cuts[k], cuts[k+1]) are the network's
+ * independent pieces, the networks of a block-diagonal batch or finer.
+ * Stops early, with an empty result, once a piece would exceed `cap`
+ * cells (the common case of one large connected network). */
+std::vector<int64_t> independent_cuts(const infoli_ctx *c, int64_t cap)
+{
+    const int64_t n = c->n;
+    // forward: no row below b reaches b or beyond
+    std::vector<int64_t> cand{0};
+    int64_t hi = -1;
+    for (int64_t b = 1; b <= n; b++) {
+        const int64_t i = b - 1;
+        hi = std::max(hi, i);
+        for (int64_t e = c->row_ptr[i]; e < c->row_ptr[i + 1]; e++) hi = std::max<int64_t>(hi, c->col[e]);
+        if (b == n || hi < b) {
+            if (b - cand.back() > cap) return {};
+            cand.push_back(b);
+        } else if (b - cand.back() > cap) {
+            return {};
+        }
+    }
+    // backward: no row at or above b reaches below b (directed networks)
+    std::vector<int64_t> sufmin(n + 1, n);
+    for (int64_t i = n - 1; i >= 0; i--) {
+        int64_t m = i;
+        for (int64_t e = c->row_ptr[i]; e < c->row_ptr[i + 1]; e++) m = std::min<int64_t>(m, c->col[e]);
+        sufmin[i] = std::min(sufmin[i + 1], m);
+    }
+    std::vector<int64_t> cuts{0};
+    for (size_t k = 1; k + 1 < cand.size(); k++)
+        if (sufmin[cand[k]] >= cand[k]) cuts.push_back(cand[k]);
+    cuts.push_back(n);
+    for (size_t k = 0; k + 1 < cuts.size(); k++)
+        if (cuts[k + 1] - cuts[k] > cap) return {};
+    return cuts;
+}
+
+struct NetsPlan {
+    std::vector<int32_t> rng;   // range bounds, nr + 1
+    int block = 0, cpt = 1;
+    double max_work = 0, total_work = 0;
+};
+
+/* the most cells one workgroup holds: 1024 threads x two cells (FP32) or
+ * one (FP64: 19 state values of 8 bytes per cell stay in registers) */
+int64_t nets_cap(const infoli_ctx *c) { return is_f32(c) ? 2048 : 1024; }
+
+/* Consecutive pieces packed into at most max(CUs, what the cell cap
+ * forces) ranges of at most nets_cap cells, minimising the largest range's
+ * work (binary search on it).  False when a piece is larger than a
+ * workgroup holds. */
+bool plan_nets(const infoli_ctx *c, NetsPlan &pl)
+{
+    const int64_t cap = nets_cap(c);
+    const std::vector<int64_t> cuts = independent_cuts(c, cap);
+    if (cuts.size() < 2) return false;
+    const size_t np = cuts.size() - 1;
+    std::vector<double> work(np);
+    std::vector<int64_t> cells(np);
+    pl.total_work = 0;
+    double wmax = 0;
+    for (size_t k = 0; k < np; k++) {
+        cells[k] = cuts[k + 1] - cuts[k];
+        if (cells[k] > cap) return false;
+        work[k] = kOdeTerms * (double)cells[k] + (double)(c->row_ptr[cuts[k + 1]] - c->row_ptr[cuts[k]]);
+        pl.total_work += work[k];
+        wmax = std::max(wmax, work[k]);
+    }
+    const auto count = [&](double T, std::vector<int32_t> *out) {
+        int64_t nr = 0, rc = 0;
+        double rw = 0;
+        if (out) out->assign(1, 0);
+        for (size_t k = 0; k < np; k++) {
+            if (rc > 0 && (rc + cells[k] > cap || rw + work[k] > T)) {
+                nr++;
+                if (out) out->push_back((int32_t)cuts[k]);
+                rc = 0;
+                rw = 0;
+            }
+            rc += cells[k];
+            rw += work[k];
+        }
+        nr++;
+        if (out) out->push_back((int32_t)c->n);
+        return nr;
+    };
+    const int64_t want = std::max<int64_t>(c->num_cus, count(pl.total_work, nullptr));
+    double lo = wmax, hi = pl.total_work;
+    for (int it = 0; it < 60 && hi - lo > 0.5; it++) {
+        const double mid = 0.5 * (lo + hi);
+        if (count(mid, nullptr) <= want) hi = mid;
+        else lo = mid;
+    }
+    count(hi, &pl.rng);
+    int64_t maxc = 0;
+    pl.max_work = 0;
+    for (size_t g = 0, k = 0; g + 1 < pl.rng.size(); g++) {
+        maxc = std::max<int64_t>(maxc, pl.rng[g + 1] - pl.rng[g]);
+        double w = 0;
+        for (; k < np && cuts[k] < pl.rng[g + 1]; k++) w += work[k];
+        pl.max_work = std::max(pl.max_work, w);
+    }
+    pl.cpt = maxc > 1024 ? 2 : 1;
+    pl.block = (int)round_up((maxc + pl.cpt - 1) / pl.cpt, 64);
+    return true;
+}
+
+/* The range table of a plan and, for every range whose lists fit beside
+ * its voltages in LDS, the lists in the kernel's LDS form: slot p of local
+ * cell i at p * stride + i for p below the range's widest slice, 16-bit
+ * range-local sender ids (padding: the cell itself), then the conductances
+ * unless they are all equal.  A cell's row uses the slots below the width
+ * of its 64-cell slice of the one-lane sliced-ELL layout, as the direct
+ * gather does. */
+template <typename T>
+int upload_nets_t(infoli_ctx *c, const NetsPlan &pl)
+{
+    const int64_t n = c->n, nr = (int64_t)pl.rng.size() - 1;
+    const int stride = pl.block * pl.cpt;
+    const size_t vbytes = 2 * (size_t)stride * sizeof(T);
+    std::vector<int32_t> deg(n);
+    for (int64_t i = 0; i < n; i++) deg[i] = (int32_t)(c->row_ptr[i + 1] - c->row_ptr[i]);
+    std::vector<int32_t> swid((n + 63) / 64, 0);   // slice widths of the one-lane layout
+    for (int64_t i = 0; i < n; i++) swid[i / 64] = std::max(swid[i / 64], deg[i]);
+    std::vector<NetsRange> rg(nr);
+    std::vector<unsigned char> blob;
+    size_t lds_max = vbytes;
+    int64_t in_lds = 0;
+    for (int64_t g = 0; g < nr; g++) {
+        NetsRange &R = rg[g];
+        R.r0 = pl.rng[g];
+        R.r1 = pl.rng[g + 1];
+        R.lds = 0;
+        R.wconst = 1;
+        R.off = 0;
+        R.wmax = 0;
+        R.pad = 0;
+        R.wc = 0.0;
+        bool first = true;
+        for (int64_t i = R.r0; i < R.r1; i++) {
+            R.wmax = std::max(R.wmax, swid[i / 64]);
+            for (int64_t e = c->row_ptr[i]; e < c->row_ptr[i + 1]; e++) {
+                const T wv = (T)c->w[e];
+                if (first) {
+                    R.wc = (double)wv;
+                    first = false;
+                } else if (wv != (T)R.wc) {
+                    R.wconst = 0;
+                }
+            }
+        }
+        const size_t ib = ((size_t)R.wmax * stride * 2 + 15) & ~(size_t)15;
+        const size_t need = vbytes + ib + (R.wconst ? 0 : (size_t)R.wmax * stride * sizeof(T));
+        if (need > kLdsBytes) continue;   // the lists stay in HBM (sliced ELL)
+        R.lds = 1;
+        R.off = (int64_t)blob.size();
+        lds_max = std::max(lds_max, need);
+        in_lds++;
+        blob.resize(blob.size() + (need - vbytes), 0);
+        uint16_t *ids = reinterpret_cast<uint16_t *>(blob.data() + R.off);
+        T *wl = reinterpret_cast<T *>(blob.data() + R.off + ib);
+        for (int64_t i = R.r0; i < R.r1; i++) {
+            const int64_t loc = i - R.r0, W = swid[i / 64];
+            for (int64_t p = 0; p < R.wmax; p++) {
+                const bool real = p < deg[i];
+                ids[p * stride + loc] = (uint16_t)(real ? c->col[c->row_ptr[i] + p] - R.r0 : loc);
+                if (!R.wconst && p < W) wl[p * stride + loc] = real ? (T)c->w[c->row_ptr[i] + p] : T(0);
+            }
+        }
+    }
+    (void)hipFree(c->d_rng);
+    c->d_rng = nullptr;
+    (void)hipFree(c->d_nets_lists);
+    c->d_nets_lists = nullptr;
+    HIPCHK(hipMalloc(&c->d_rng, sizeof(NetsRange) * rg.size()));
+    RCHK(h2d(c, c->d_rng, rg.data(), sizeof(NetsRange) * rg.size()));
+    HIPCHK(hipMalloc(&c->d_nets_lists, std::max<size_t>(blob.size(), 16)));
+    RCHK(h2d(c, c->d_nets_lists, blob.data(), blob.size()));
+    c->nets_nr = nr;
+    c->nets_block = pl.block;
+    c->nets_cpt = pl.cpt;
+    c->nets_lds = lds_max;
+    c->nets_in_lds = in_lds;
+    c->nets_valid = true;
+    return INFOLI_OK;
+}
+
+int upload_nets(infoli_ctx *c)
+{
+    NetsPlan pl;
+    if (!plan_nets(c, pl))
+        return infoli_fail(INFOLI_ERR_ARG, "gather NETS: a piece of the network that no junction separates has "
+                                           "more than %lld cells, more than one workgroup holds",
+                           (long long)nets_cap(c));
+    if (pl.block * pl.cpt > 65536)
+        return infoli_fail(INFOLI_ERR_ARG, "gather NETS: range of %d cells", pl.block * pl.cpt);
+    return is_f32(c) ? upload_nets_t<float>(c, pl) : upload_nets_t<double>(c, pl);
+}
+
 int ensure_layout(infoli_ctx *c)
 {
     const int L = c->exec.lanes_per_cell;
@@ -963,6 +1218,11 @@ int ensure_layout(infoli_ctx *c)
                        : upload_tiled<double>(c, L, c->exec.tile_cells, c->exec.lds_kb));
     }
     if (c->exec.gather == GATHER_TILED) c->tl_nsa = soma_axon_workgroups(c);
+    if (c->exec.gather == GATHER_NETS && !c->nets_valid) {
+        HIPCHK(hipSetDevice(c->device));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        RCHK(upload_nets(c));
+    }
     return INFOLI_OK;
 }
 
@@ -1033,6 +1293,25 @@ void default_exec(infoli_ctx *c)
     c->exec.ode_split = 1;
     c->exec.fast_math = 0;
     c->exec.gather = GATHER_DIRECT;
+    // whole networks per workgroup, persistent, when every piece of the
+    // network that no junction separates fits one workgroup and the largest
+    // range's work, advanced with no launch between steps, beats the graph's
+    // estimate: the whole work spread over the CUs plus the two dependent
+    // kernels of a step
+    {
+        NetsPlan pl;
+        if (!hyb_active(c) && plan_nets(c, pl)) {
+            const double persist_us = pl.max_work * kNetsUsPerTerm;
+            const double graph_us = pl.total_work * kGraphUsPerTerm / std::max(c->num_cus, 1) + kGraphStepUs;
+            if (persist_us < graph_us) {
+                c->exec.mode = INFOLI_EXEC_PERSISTENT;
+                c->exec.gather = GATHER_NETS;
+                c->exec.lanes_per_cell = 1;
+                c->exec.steps_per_launch = 500;
+                return;
+            }
+        }
+    }
     if (mean_deg < 4 && !hyb_active(c)) return;
     // tiled: receivers per tile from the mean in-tile degree of a first
     // layout guess (2048 / L0 with L0 about one lane per 16-32 junctions),
@@ -1353,6 +1632,8 @@ extern "C" void infoli_destroy(infoli_t *c)
         if (b) (void)hipFree(b);
     free_staged(c);
     free_tiled(c);
+    if (c->d_rng) (void)hipFree(c->d_rng);
+    if (c->d_nets_lists) (void)hipFree(c->d_nets_lists);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
     if (c->h_status) (void)hipHostFree(c->h_status);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1362,6 +1643,8 @@ extern "C" void infoli_destroy(infoli_t *c)
 /* ------------------------------------------------------------------------ */
 /* inputs                                                                    */
 /* ------------------------------------------------------------------------ */
+
+int check_exec(const infoli_ctx *c, infoli_exec &x);
 
 extern "C" int infoli_set_connectivity_csr(infoli_t *c, const int64_t *row_ptr,
                                            const int32_t *col, const double *w)
@@ -1403,9 +1686,16 @@ extern "C" int infoli_set_connectivity_csr(infoli_t *c, const int64_t *row_ptr,
     c->max_deg = max_deg;
     c->staged_valid = false;
     c->tiled_valid = false;
-    const int keep_mode = c->exec.mode;
+    c->nets_valid = false;
     default_exec(c);
-    if (keep_mode != INFOLI_EXEC_GRAPH) c->exec.mode = keep_mode;
+    // a launch mode the caller chose is kept where the new strategy has it
+    // (whole networks per workgroup are persistent by construction)
+    if (c->user_mode >= 0 && c->exec.gather != GATHER_NETS) {
+        infoli_exec x = c->exec;
+        x.mode = c->user_mode;
+        if (check_exec(c, x) == INFOLI_OK) c->exec = x;
+        infoli_clear_error();
+    }
     c->sell_L = 0;
     drop_graph(c);
     return ensure_layout(c);
@@ -1683,11 +1973,12 @@ int check_exec(const infoli_ctx *c, infoli_exec &x)
         return infoli_fail(INFOLI_ERR_ARG, "lanes_per_cell %d not in {1,2,4,8,16}", x.lanes_per_cell);
     if (x.gather == GATHER_SWEEP) x.block = 1024;       // one workgroup of 1024 threads per receiver block
     else if (x.gather != GATHER_TILED) x.block = kBlock;   // the fused step kernel has one block size
+    if (x.gather == GATHER_NETS) x.lanes_per_cell = 1;    // one thread per cell (the workgroup size is planned)
     if (x.block != 256 && x.block != 512 && x.block != 1024)
         return infoli_fail(INFOLI_ERR_ARG, "block %d: the tiled gather takes 256, 512 or 1024", x.block);
     if (x.steps_per_launch < 1 || x.steps_per_launch > 100000)
         return infoli_fail(INFOLI_ERR_ARG, "steps_per_launch %d out of range", x.steps_per_launch);
-    if (x.gather < GATHER_DIRECT || x.gather > GATHER_SWEEP)
+    if (x.gather < GATHER_DIRECT || x.gather > GATHER_NETS)
         return infoli_fail(INFOLI_ERR_ARG, "infoli_set_exec: gather %d unknown", x.gather);
     if (tiled_layout(x) && (x.tile_cells < 64 || x.tile_cells > 16384 || x.tile_cells % 64))
         return infoli_fail(INFOLI_ERR_ARG, "tile_cells %d: must be a multiple of 64 in [64, 16384]",
@@ -1700,6 +1991,17 @@ int check_exec(const infoli_ctx *c, infoli_exec &x)
         return infoli_fail(INFOLI_ERR_ARG, "cells with at least %lld neighbours sum FP64 gap-junction terms, "
                                            "which the tiled gathers implement; set the FP64 degree to 0 "
                                            "for another gather", (long long)c->fp64_deg);
+    if (x.gather == GATHER_NETS) {
+        if (x.mode != INFOLI_EXEC_PERSISTENT)
+            return infoli_fail(INFOLI_ERR_ARG, "gather NETS (whole networks per workgroup) runs in the persistent "
+                                               "launch mode only");
+        if (hyb_active(c))
+            return infoli_fail(INFOLI_ERR_ARG, "gather NETS sums FP32 terms only; set the FP64 degree to 0");
+        if (independent_cuts(c, nets_cap(c)).empty())
+            return infoli_fail(INFOLI_ERR_ARG, "gather NETS: a piece of the network that no junction separates has "
+                                               "more than %lld cells, more than one workgroup holds",
+                               (long long)nets_cap(c));
+    }
     if (tiled_layout(x) && x.mode == INFOLI_EXEC_PERSISTENT)
         return infoli_fail(INFOLI_ERR_ARG, "the tiled gathers run as two kernels per step; "
                                            "use the step or graph launch mode");
@@ -1747,6 +2049,7 @@ extern "C" int infoli_set_exec(infoli_t *c, const infoli_exec *e)
     drop_graph(c);
     int rc = ensure_layout(c);
     if (rc != INFOLI_OK) c->exec = prev;
+    else c->user_mode = e->mode == INFOLI_EXEC_AUTO ? -1 : x.mode;
     return rc;
 }
 
@@ -2208,8 +2511,9 @@ extern "C" int infoli_autotune(infoli_t *c, int64_t trial_steps, infoli_exec *ch
 
     // candidates: the heuristic's choice first, then tiled shapes around it
     // (lanes, receivers per tile and chunk size halved and doubled, both
-    // compartment placements), swept tiles, then the other gathers unless
-    // some cell needs the FP64 sums only the tiled gathers have
+    // compartment placements), swept tiles, whole networks per workgroup,
+    // then the other gathers unless some cell needs the FP64 sums only the
+    // tiled gathers have
     std::vector<infoli_exec> cands;
     const int spl = (int)std::min<int64_t>(trial_steps, 500);
     default_exec(c);
@@ -2252,6 +2556,11 @@ extern "C" int infoli_autotune(infoli_t *c, int64_t trial_steps, infoli_exec *ch
                 if (memcmp(&e, &h, sizeof e) != 0) cands.push_back(e);
             }
         }
+    }
+    // whole networks per workgroup, when every piece of the network fits one
+    if (!hyb_active(c) && !independent_cuts(c, nets_cap(c)).empty()) {
+        const infoli_exec e{INFOLI_EXEC_PERSISTENT, 1, kBlock, spl, GATHER_NETS, kDefaultLdsKb, 1024, 1, 0};
+        if (memcmp(&e, &h, sizeof e) != 0) cands.push_back(e);
     }
     if (!hyb_active(c)) {
         const int modes[] = {INFOLI_EXEC_GRAPH, INFOLI_EXEC_PERSISTENT};

@@ -127,8 +127,10 @@ enum Gather {
     GATHER_STAGED = 2,   // per block: senders sorted, copied to LDS, then read in row order
     GATHER_TILED = 3,    // receiver block x sender chunk tiles, chunk of V_dend in LDS,
                          // partial sums merged by the compartment kernel
-    GATHER_SWEEP = 4     // the tiled layout, one workgroup per receiver block walking its
+    GATHER_SWEEP = 4,    // the tiled layout, one workgroup per receiver block walking its
                          // chunks, the partial sums merged in LDS
+    GATHER_NETS = 5      // whole networks per workgroup, V_dend in LDS and the state in
+                         // registers for many steps per launch (k_persist_nets)
 };
 
 struct GridBarrier {
@@ -675,7 +677,8 @@ constexpr int kUnroll = 8;   // row entries in flight per lane
 /* The lane owns row positions p = kk*L + l (kk = 0, 1, ...) of a row of d
  * entries; entry kk sits at base + kk*64.  W = the slice width, uniform
  * across the wavefront, so every slot kk < W is allocated: slots past the
- * row's end are padding (sender 0, conductance 0) and are loaded like the
+ * row's end are padding (sender = the row's own cell, conductance 0) and are
+ * loaded like the
  * others -- no divergent branches, kUnroll loads in flight -- while their
  * term is replaced by an exact +0 (a no-op for the plain and the TwoSum
  * accumulator).  Terms are accumulated in position order. */
@@ -928,6 +931,173 @@ __global__ __launch_bounds__(kBlock) void k_persist(StepArgs<T> a, GridBarrier *
         for (int64_t b = blockIdx.x; b < a.nblocks; b += gridDim.x)
             step_block<T, L, KAHAN, G>(a, b, t0 + k, ic_lds, reinterpret_cast<T *>(dyn_lds));
         if (k + 1 < nsteps && !grid_barrier(gb, (unsigned)(k + 1) * gridDim.x)) return;
+    }
+}
+
+/* One workgroup's cell range of GATHER_NETS: cells [r0, r1), whole
+ * networks.  With lds, the range's neighbour lists live in LDS for the whole
+ * launch, copied from `off` in the lists buffer: slot p of local cell i at
+ * p * stride + i, wmax slots, 16-bit range-local sender ids, then the
+ * conductances (none when every conductance of the range is wc). */
+struct NetsRange {
+    int32_t r0, r1;
+    int32_t lds, wconst;
+    int64_t off;
+    int32_t wmax, pad;
+    double wc;
+};
+
+/* Whole networks per workgroup, many steps per launch (block-diagonal
+ * batches of small networks).  Workgroup g owns rg[g]: whole networks, so
+ * every neighbour of its cells is its own.  It keeps their V_dend in LDS
+ * (two halves, by step parity), their state in registers and, when they
+ * fit, their neighbour lists in LDS, and advances nsteps steps with nothing
+ * but workgroup barriers between them: no other workgroup's data is read, so
+ * no cross-workgroup hand-over (which costs more than a kernel boundary on
+ * this GPU) is needed.  Thread i holds local cells i + q * blockDim
+ * (q < CPT); stride = blockDim * CPT.  Each row is summed in list order up to
+ * the width of its 64-cell slice of the one-lane sliced-ELL layout, padding
+ * slots adding +0, exactly as the direct gather sums it (row_terms_idx with
+ * L = 1), and each cell is updated by `ode`, so the results are those of the
+ * other strategies at one lane per cell, bit for bit.  The state returns to
+ * HBM after the last step; the V_axon trace is written on the trace steps as
+ * usual. */
+template <typename T, bool KAHAN, int CPT>
+__global__ __launch_bounds__(1024) void k_persist_nets(StepArgs<T> a, const NetsRange *__restrict__ rg,
+                                                       const unsigned char *__restrict__ lists, int nsteps)
+{
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
+    const int nt = blockDim.x, tid = threadIdx.x, stride = nt * CPT;
+    const NetsRange R = rg[blockIdx.x];
+    const int64_t r0 = R.r0;
+    const int nc = R.r1 - R.r0;
+    T *v0 = reinterpret_cast<T *>(dyn_lds);
+    T *v1 = v0 + stride;
+    const uint16_t *ids = reinterpret_cast<const uint16_t *>(v1 + stride);
+    const T *wl = reinterpret_cast<const T *>(dyn_lds + 2 * (size_t)stride * sizeof(T) +
+                                              (((size_t)R.wmax * stride * 2 + 15) & ~(size_t)15));
+    if (R.lds) {
+        // the range's lists into LDS, 16 bytes per thread and load
+        const size_t bytes = (((size_t)R.wmax * stride * 2 + 15) & ~(size_t)15) +
+                             (R.wconst ? 0 : (size_t)R.wmax * stride * sizeof(T));
+        const uint4 *src = reinterpret_cast<const uint4 *>(lists + R.off);
+        uint4 *dst = reinterpret_cast<uint4 *>(v1 + stride);
+        for (size_t i = tid; i < bytes / 16; i += nt) dst[i] = src[i];
+    }
+    const T wc = (T)R.wc;
+    const int64_t t0 = *a.d_step;
+    typename CellOf<T>::type cs[CPT];
+    T ib[CPT];
+    bool mk[CPT];
+    int dg[CPT], W[CPT];
+    int64_t base[CPT];
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+        const int loc = tid + q * nt;
+        const int64_t cell = r0 + loc;
+        dg[q] = 0;
+        W[q] = 0;
+        base[q] = 0;
+        ib[q] = T(0);
+        mk[q] = false;
+        if (loc < nc) {
+            load_cell(a, cell, t0, cs[q]);
+            v0[loc] = cs[q].vd;
+            ib[q] = a.iapp_base ? a.iapp_base[cell] : T(0);
+            mk[q] = !a.mask || a.mask[cell];
+            dg[q] = a.deg[cell];
+            const int64_t s_ = cell >> 6;
+            const int64_t b0 = a.slice_ptr[s_];
+            W[q] = (int)((a.slice_ptr[s_ + 1] - b0) >> 6);
+            base[q] = b0 + (cell & 63);
+        }
+    }
+    __syncthreads();
+    for (int k = 0; k < nsteps; k++) {
+        const int64_t t = t0 + k;
+        const T *__restrict__ vc = (k & 1) ? v1 : v0;
+        T *__restrict__ vn = (k & 1) ? v0 : v1;
+#pragma unroll
+        for (int q = 0; q < CPT; q++) {
+            const int loc = tid + q * nt;
+            if (loc < nc) {
+                const int64_t cell = r0 + loc;
+                Acc<T, KAHAN> acc;
+                if (dg[q] > 0) {
+                    const T vi = vc[loc];
+                    if (R.lds) {
+                        // the row from LDS: batches of kUnroll slots, as row_terms_idx
+                        const int d = dg[q], Wq = W[q];
+                        for (int kk0 = 0; kk0 < Wq; kk0 += kUnroll) {
+                            int jv[kUnroll];
+                            T wv[kUnroll], vv[kUnroll];
+#pragma unroll
+                            for (int u = 0; u < kUnroll; u++) {
+                                if (kk0 + u < Wq) {
+                                    jv[u] = ids[(kk0 + u) * stride + loc];
+                                    wv[u] = R.wconst ? wc : wl[(kk0 + u) * stride + loc];
+                                }
+                            }
+#pragma unroll
+                            for (int u = 0; u < kUnroll; u++)
+                                if (kk0 + u < Wq) vv[u] = vc[jv[u]];
+#pragma unroll
+                            for (int u = 0; u < kUnroll; u++) {
+                                if (kk0 + u < Wq) {
+                                    const T term = gj_term(vi, vv[u], wv[u]);
+                                    acc.add((kk0 + u) < d ? term : T(0));
+                                }
+                            }
+                        }
+                    } else {
+                        row_terms_idx<T, KAHAN, 1, false>(acc, vi, dg[q], 0, W[q], base[q], a.w, a.col,
+                                                          [&](int j) { return vc[j - r0]; });
+                    }
+                }
+                // iapp_of, with this cell's constant and mask in registers
+                T iapp;
+                if (a.tab && t >= a.tab_t0 && t < a.tab_t0 + a.tab_n)
+                    iapp = a.tab[(t - a.tab_t0) * a.n + cell];
+                else
+                    iapp = ((t >= a.t0) && (t < a.t1) && mk[q]) ? ib[q] + a.amp : ib[q];
+                ode(a.m, cs[q], acc.value(), iapp);
+                vn[loc] = cs[q].vd;
+                if (a.trace && (t % a.stride) == 0)
+                    a.trace[((t / a.stride) % a.trace_cap) * a.n + cell] = cs[q].va;
+            }
+        }
+        __syncthreads();
+    }
+    // the state after the last step, as store_cell leaves it (no trace)
+    const int64_t tl = t0 + nsteps - 1;
+    const int nx = (int)((tl & 1) ^ 1);
+    const int64_t np = a.npad;
+    T *st = a.st;
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+        const int loc = tid + q * nt;
+        if (loc < nc && nsteps > 0) {
+            const int64_t cell = r0 + loc;
+            const typename CellOf<T>::type &c = cs[q];
+            a.vd[nx][cell] = c.vd;
+            a.vs[nx][cell] = c.vs;
+            a.va[nx][cell] = c.va;
+            st[CALCIUM_R * np + cell] = c.r;
+            st[POTASSIUM_S * np + cell] = c.s;
+            st[HCURRENT_Q * np + cell] = c.q;
+            st[CA2PLUS * np + cell] = c.ca;
+            st[I_CAH * np + cell] = c.icah;
+            st[SODIUM_M * np + cell] = c.m;
+            st[SODIUM_H * np + cell] = c.h;
+            st[POTASSIUM_N * np + cell] = c.n;
+            st[POTASSIUM_P * np + cell] = c.p;
+            st[POTASSIUM_X_S * np + cell] = c.xs;
+            st[CALCIUM_K * np + cell] = c.k;
+            st[CALCIUM_L * np + cell] = c.l;
+            st[SODIUM_M_A * np + cell] = c.ma;
+            st[SODIUM_H_A * np + cell] = c.ha;
+            st[POTASSIUM_X_A * np + cell] = c.xa;
+        }
     }
 }
 

@@ -18,7 +18,7 @@ FP64_DEGREE_DEFAULT = 1280   # FP32_KAHAN: cells of at least this degree sum FP6
 _PREC_NAMES = {"fp64": FP64, "fp32": FP32, "fp32k": FP32_KAHAN, "fp32_kahan": FP32_KAHAN}
 
 EXEC_AUTO, EXEC_STEP, EXEC_GRAPH, EXEC_PERSISTENT = 0, 1, 2, 3
-GATHER_DIRECT, GATHER_LDS, GATHER_STAGED, GATHER_TILED, GATHER_SWEEP = 0, 1, 2, 3, 4
+GATHER_DIRECT, GATHER_LDS, GATHER_STAGED, GATHER_TILED, GATHER_SWEEP, GATHER_NETS = 0, 1, 2, 3, 4, 5
 
 # per-cell variables, bin/default.conf lines 24-42 order
 VARS = ("V_dend", "Calcium_r", "Potassium_s", "Hcurrent_q", "Ca2Plus", "I_CaH",
@@ -27,7 +27,7 @@ VARS = ("V_dend", "Calcium_r", "Potassium_s", "Hcurrent_q", "Ca2Plus", "I_CaH",
         "Sodium_h_a", "Potassium_x_a")
 _VAR_INDEX = {v: i for i, v in enumerate(VARS)}
 
-ERR_NONFINITE = -5
+ERR_ARG, ERR_STATE, ERR_NONFINITE = -1, -4, -5
 
 
 class InfoliError(RuntimeError):
@@ -355,6 +355,10 @@ class Network:
     def set_exec(self, mode=None, lanes_per_cell=None, block=None, steps_per_launch=None,
                  gather=None, lds_kb=None, tile_cells=None, ode_split=None, fast_math=None):
         cur = self.exec
+        if gather is not None and gather != GATHER_NETS and mode is None and cur["gather"] == GATHER_NETS:
+            # leaving whole networks per workgroup (persistent by construction):
+            # the other gathers start from graph launches
+            cur["mode"] = EXEC_GRAPH
         for k, v in (("mode", mode), ("lanes_per_cell", lanes_per_cell), ("block", block),
                      ("steps_per_launch", steps_per_launch), ("gather", gather),
                      ("lds_kb", lds_kb), ("tile_cells", tile_cells), ("ode_split", ode_split),

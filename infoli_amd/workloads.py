@@ -19,6 +19,14 @@ run see bit-identical inputs.  Streams: 0 graph, 1 w, 2 g_CaL, 3 Iapp,
   large_1m     1,000,000 cells, G(n, 50/(n-1)), w ~ U[0.02, 0.06],
                g_CaL ~ U[0.5, 1.1], Iapp ~ N(1, 1) constant, 2,000 steps.
 
+And a batch of small networks, the shape a parameter sweep takes:
+
+  sweep_256    a parameter sweep of small networks: 16 x 16 networks of
+               256 cells, G(256, 0.1), w = 0.04, g_CaL on a 16-point grid
+               of [0.5, 1.1] and constant Iapp on a 16-point grid of
+               [-1, 3] uA/cm^2, 20,000 steps (the batching the reference's
+               per-process runs would need 256 processes for).
+
 And one network for the precision statement where the gap junctions act
 on spikes (the five above either stay subthreshold or, in plain_1k and
 batched_512, keep every cell of a network identical, so I_c = 0):
@@ -39,7 +47,7 @@ from .network import gen_mask, gen_normal, gen_symmetric, gen_uniform
 MASTER_SEED = 20261203
 S_GRAPH, S_W, S_GCAL, S_IAPP, S_MASK, S_SIZES = 0, 1, 2, 3, 4, 5
 
-NAMES = ("plain_1k", "dense_10k", "sparse_100k", "batched_512", "large_1m", "spiking_20k")
+NAMES = ("plain_1k", "dense_10k", "sparse_100k", "batched_512", "large_1m", "spiking_20k", "sweep_256")
 
 
 def _net_seed(seed, k):
@@ -104,6 +112,32 @@ def batched_512(n_nets=512, lo=64, hi=4096, steps=20000, seed=MASTER_SEED, p=0.1
                     partition=offs, sizes=sizes)
 
 
+def sweep_256(grid=16, size=256, steps=20000, seed=MASTER_SEED, p=0.1):
+    """A parameter sweep of small networks: grid x grid networks of `size`
+    cells, network (a, b) with g_CaL = 0.5 + 0.6 a / (grid - 1) and a
+    constant Iapp = -1 + 4 b / (grid - 1) uA/cm^2, each G(size, p) with
+    w = 0.04."""
+    n_nets = grid * grid
+    offs = np.arange(n_nets + 1, dtype=np.int64) * size
+    rps, cols, ws = [np.zeros(1, np.int64)], [], []
+    base_nnz = 0
+    for k in range(n_nets):
+        rp, col, w = gen_symmetric(_net_seed(seed, k), size, p, 0.04)
+        rps.append(rp[1:] + base_nnz)
+        cols.append(col + offs[k])
+        ws.append(w)
+        base_nnz += int(rp[-1])
+    a = np.repeat(np.arange(grid), grid)
+    b = np.tile(np.arange(grid), grid)
+    gnet = 0.5 + 0.6 * a / (grid - 1)
+    inet = -1.0 + 4.0 * b / (grid - 1)
+    sizes = np.full(n_nets, size, np.int64)
+    return Workload(name="sweep_256", n=int(offs[-1]), row_ptr=np.concatenate(rps),
+                    col=np.concatenate(cols).astype(np.int32), w=np.concatenate(ws),
+                    gcal=np.repeat(gnet, size), iapp_base=np.repeat(inet, size), mask=None, amp=0.0,
+                    t0=0, t1=0, steps=steps, partition=offs, sizes=sizes)
+
+
 def large_1m(n=1000000, steps=2000, seed=MASTER_SEED, mean_deg=50.0):
     rp, col, w = gen_symmetric(seed, n, mean_deg / (n - 1), 0.02, 0.06)
     return Workload(name="large_1m", n=n, row_ptr=rp, col=col, w=w,
@@ -121,7 +155,8 @@ def spiking_20k(n=20000, steps=6000, seed=MASTER_SEED, mean_deg=50.0):
 
 
 BUILDERS = {"plain_1k": plain_1k, "dense_10k": dense_10k, "sparse_100k": sparse_100k,
-            "batched_512": batched_512, "large_1m": large_1m, "spiking_20k": spiking_20k}
+            "batched_512": batched_512, "large_1m": large_1m, "spiking_20k": spiking_20k,
+            "sweep_256": sweep_256}
 
 
 def build(name, **kw):

@@ -293,13 +293,13 @@ def test_swept_tiles_options_checked():
     say("swept tiles option checks")
     net = make_net(40000, "fp32k")
     with pytest.raises(I.InfoliError):   # no somata and axons beside the swept tiles
-        net.set_exec(gather=I.GATHER_SWEEP, tile_cells=512, lds_kb=8, ode_split=2)
+        net.set_exec(mode=I.EXEC_GRAPH, gather=I.GATHER_SWEEP, tile_cells=512, lds_kb=8, ode_split=2)
     with pytest.raises(I.InfoliError):   # 16384 sums (128 KiB) and a 39 KiB chunk exceed the LDS
-        net.set_exec(gather=I.GATHER_SWEEP, tile_cells=16384, lds_kb=40, ode_split=1)
+        net.set_exec(mode=I.EXEC_GRAPH, gather=I.GATHER_SWEEP, tile_cells=16384, lds_kb=40, ode_split=1)
     with pytest.raises(I.InfoliError):
         net.set_exec(mode=I.EXEC_PERSISTENT, gather=I.GATHER_SWEEP, tile_cells=512, lds_kb=8, ode_split=1)
     with pytest.raises(I.InfoliError):
-        net.set_exec(gather=I.GATHER_SWEEP + 1)
+        net.set_exec(gather=I.GATHER_NETS + 1)
     net.set_exec(mode=I.EXEC_GRAPH, gather=I.GATHER_SWEEP, tile_cells=16384, lds_kb=32, ode_split=1,
                  block=256)
     e = net.exec

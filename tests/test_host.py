@@ -57,7 +57,8 @@ def test_python_constants_match_header():
             for m in re.finditer(r"#define INFOLI_([A-Z0-9_]+)\s+(-?\d+)", hdr)}
     names = ["FP64", "FP32", "FP32_KAHAN", "FP64_DEGREE_DEFAULT",
              "EXEC_AUTO", "EXEC_STEP", "EXEC_GRAPH", "EXEC_PERSISTENT",
-             "GATHER_DIRECT", "GATHER_LDS", "GATHER_STAGED", "GATHER_TILED", "GATHER_SWEEP"]
+             "GATHER_DIRECT", "GATHER_LDS", "GATHER_STAGED", "GATHER_TILED", "GATHER_SWEEP",
+             "GATHER_NETS", "ERR_ARG", "ERR_STATE", "ERR_NONFINITE"]
     for name in names:
         assert getattr(I, name) == defs[name], name
 
