@@ -401,6 +401,13 @@ const void *const k_tile_sweep_tab[3][5] = {
     INFOLI_L_TABLE(k_tile_sweep, float, true, 1024),
 };
 const void *const k_tile_sweep_hyb_tab[5] = INFOLI_L_TABLE(k_tile_sweep, float, true, 1024, true);
+/* blocks of fewer slices than wavefronts: the compartments beside the rows */
+const void *const k_tile_sweep_small_tab[3][5] = {
+    INFOLI_L_TABLE(k_tile_sweep, double, false, 1024, false, true),
+    INFOLI_L_TABLE(k_tile_sweep, float, false, 1024, false, true),
+    INFOLI_L_TABLE(k_tile_sweep, float, true, 1024, false, true),
+};
+const void *const k_tile_sweep_small_hyb_tab[5] = INFOLI_L_TABLE(k_tile_sweep, float, true, 1024, true, true);
 /* [fast math][precision] */
 const void *const k_tile_dend_tab[2][3] = {
     {(const void *)k_tile_dend<double, false, false>, (const void *)k_tile_dend<float, false, false>,
@@ -462,6 +469,11 @@ const void *tile_kernel(const infoli_ctx *c)
 const void *sweep_kernel(const infoli_ctx *c)
 {
     const int li = lidx(c->exec.lanes_per_cell);
+    // a block of fewer slices than the workgroup's 16 wavefronts, with the
+    // compartments in the sweeping kernel: the idle wavefronts run the
+    // somata, axons and dendrite gates beside the rows
+    if (c->exec.ode_split == 0 && c->exec.tile_cells / (64 / c->exec.lanes_per_cell) < 16)
+        return hyb_active(c) ? k_tile_sweep_small_hyb_tab[li] : k_tile_sweep_small_tab[c->prec][li];
     return hyb_active(c) ? k_tile_sweep_hyb_tab[li] : k_tile_sweep_tab[c->prec][li];
 }
 
@@ -1310,6 +1322,26 @@ void default_exec(infoli_ctx *c)
                 c->exec.steps_per_launch = 500;
                 return;
             }
+        }
+    }
+    // a small FP32 network (at most 2,048 cells, so one sender chunk and at
+    // most 32 blocks): swept tiles of 64 receivers, about one lane per 16
+    // junctions, the compartments in the sweeping kernel beside the rows --
+    // one kernel per step.  plain_1k: 6.97 us per step at L 8 (L 4 7.26,
+    // L 16 9.33, R 128 L 4 7.42) against 9.3 for the tiled layout's four
+    // tiles and its two dependent kernels (FP64: 14.7 against 14.2, kept tiled)
+    if (is_f32(c) && c->n <= 2048 && mean_deg >= 1.0 && !hyb_active(c)) {
+        int L = 1;
+        while (L < 16 && 16.0 * L <= mean_deg) L *= 2;
+        const int kb = (int)std::max<int64_t>(1, ((int64_t)c->n * (int64_t)c->tsize + 1023) / 1024);
+        if (sweep_lds_bytes(c, 64, tiled_chunk(c, kb)) <= kLdsBytes) {
+            c->exec.gather = GATHER_SWEEP;
+            c->exec.lanes_per_cell = L;
+            c->exec.block = 1024;
+            c->exec.tile_cells = 64;
+            c->exec.lds_kb = kb;
+            c->exec.ode_split = 0;
+            return;
         }
     }
     if (mean_deg < 4 && !hyb_active(c)) return;

@@ -1625,9 +1625,10 @@ __global__ __launch_bounds__(NT, (sizeof(T) == 4 && !HYB) ? (AWC ? 4 : 8) : 1) v
  * The terms are added in slot order with the checks of
  * tile_row_terms, so each row's pair is the one tile_rows makes, and it is
  * merged into the receiver's accumulator in LDS. */
-template <typename T, int L, bool KAHAN, int NT, bool HYB>
+template <typename T, int L, bool KAHAN, int NT, bool HYB, typename Idle>
 __device__ __forceinline__ void sweep_tile_rows(const StepArgs<T> &a, const TileDesc td, const T *__restrict__ vin,
-                                                T *__restrict__ vch, typename PartOf<T, KAHAN>::type *accl)
+                                                T *__restrict__ vch, typename PartOf<T, KAHAN>::type *accl,
+                                                Idle &&idle)
 {
     constexpr int CPW = 64 / L, NW = NT / 64;
     const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -1683,6 +1684,9 @@ __device__ __forceinline__ void sweep_tile_rows(const StepArgs<T> &a, const Tile
     for (int i = 0; i < DB; i++) load_b(snake(i), av[i], bv[i]);
     tile_chunk_to_lds<T, NT>(vin + td.s0, vch, td.cn, tid);
     __syncthreads();
+    // wavefronts without a slice of this tile (a block of few slices)
+    // take other work while the rows are summed
+    if (nsl < NW && wave >= nsl) idle(wave - nsl, NW - nsl);
     for (int r = 0; r * NW < nsl; r++) {
         const int sl = snake(r);
         load_a(snake(r + DB + 1), av[DB + 1]);
@@ -1770,7 +1774,7 @@ __device__ __forceinline__ void sweep_tile_rows(const StepArgs<T> &a, const Tile
  * networks whose blocks draw on many chunks: the per-tile partials would
  * cross HBM twice, here they stay in LDS.  LDS: R accumulators, then the
  * chunk. */
-template <typename T, int L, bool KAHAN, int NT, bool HYB = false>
+template <typename T, int L, bool KAHAN, int NT, bool HYB = false, bool SMALL = false>
 __global__ __launch_bounds__(NT, 1) void k_tile_sweep(StepArgs<T> a, int64_t k)
 {
     extern __shared__ __align__(16) unsigned char dyn_lds[];
@@ -1785,13 +1789,72 @@ __global__ __launch_bounds__(NT, 1) void k_tile_sweep(StepArgs<T> a, int64_t k)
         else accl[r] = T(0);
     }
     const int32_t t0 = a.rb_tile0[rb], t1 = a.rb_tile0[rb + 1];
+    // A block of fewer slices than wavefronts (small networks; the SMALL
+    // instantiation) leaves wavefronts idle during the rows of its first
+    // tile: with sweep_ode they
+    // run the block's somata, axons and dendrites up to their voltage there
+    // (none needs this step's I_c), and only the dendrite voltages follow the
+    // sums, as after the stored tiles with ode_split 2.
+    const int64_t cell0 = rb * a.rcells;
+    const int64_t ncell = a.n - cell0 < a.rcells ? a.n - cell0 : a.rcells;
+    const bool beside = SMALL && a.sweep_ode && a.rcells / (64 / L) < NT / 64 && t1 > t0;
+    const auto idle = [&](int iw, int nw) {
+        if (!beside) return;
+        const int lane = threadIdx.x & 63;
+        const int ng = (int)((ncell + 63) >> 6);   // task g < ng: somata of group g; ng + g: axons, dendrite gates
+        for (int task = iw; task < 2 * ng; task += nw) {
+            const int g = task < ng ? task : task - ng;
+            const int64_t cell = cell0 + g * 64 + lane;
+            if (cell < a.n) {
+                if (a.fast_math) {
+                    if (task < ng) tile_soma_cell<T, true>(a, cell, t);
+                    else {
+                        tile_axon_cell<T, true>(a, cell, t);
+                        tile_dend_pre_cell<T, true>(a, cell, t);
+                    }
+                } else {
+                    if (task < ng) tile_soma_cell<T, false>(a, cell, t);
+                    else {
+                        tile_axon_cell<T, false>(a, cell, t);
+                        tile_dend_pre_cell<T, false>(a, cell, t);
+                    }
+                }
+            }
+        }
+    };
     for (int32_t ti = t0; ti < t1; ti++) {
         // the last tile's rows are done with the chunk buffer (and, before
         // the first tile, the accumulators are cleared)
         __syncthreads();
-        sweep_tile_rows<T, L, KAHAN, NT, HYB>(a, a.tiles[ti], vin, vch, accl);
+        if constexpr (SMALL) {
+            if (ti == t0) sweep_tile_rows<T, L, KAHAN, NT, HYB>(a, a.tiles[ti], vin, vch, accl, idle);
+            else sweep_tile_rows<T, L, KAHAN, NT, HYB>(a, a.tiles[ti], vin, vch, accl, [](int, int) {});
+        } else {
+            sweep_tile_rows<T, L, KAHAN, NT, HYB>(a, a.tiles[ti], vin, vch, accl, [](int, int) {});
+        }
     }
     __syncthreads();
+    if (beside) {
+        // the dendrite voltages from the merged sums (I_c as below)
+        const int cur = (int)(t & 1), nxt = cur ^ 1;
+        for (int r = threadIdx.x; r < ncell; r += NT) {
+            const int64_t cell = cell0 + r;
+            T ic;
+            if constexpr (KAHAN) {
+                const float2 v = accl[r];
+                ic = v.x + v.y;
+            } else {
+                ic = accl[r];
+            }
+            if (a.fast_math)
+                a.vd[nxt][cell] = dend_post<true>(a.m, a.vd[cur][cell], a.dpre[cell], a.dpre[a.npad + cell], ic,
+                                                  iapp_of(a, cell, t));
+            else
+                a.vd[nxt][cell] = dend_post<false>(a.m, a.vd[cur][cell], a.dpre[cell], a.dpre[a.npad + cell], ic,
+                                                   iapp_of(a, cell, t));
+        }
+        return;
+    }
     if (a.sweep_ode) {
         // the block's cells, one thread each, with I_c = the merged pair's
         // value (what tiled_ic returns for the stored pair)

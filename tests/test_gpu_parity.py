@@ -243,6 +243,42 @@ def test_swept_tiles_equal_tiles(prec, fm, L, R, kb, split):
         assert np.abs(outs[1][0] - ref).max() < (1e-10 if prec == "fp64" else 2e-4)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "fp32k"])
+@pytest.mark.parametrize("L", [1, 4, 8])
+def test_small_network_sweep_compartments_beside_rows(prec, L):
+    """A network of 1,000 cells (heterogeneous g_CaL and Iapp): the default
+    FP32 strategy (set explicitly in FP64) sweeps 64-receiver blocks with the
+    somata, axons and dendrite gates in the idle wavefronts beside the rows;
+    at each lane count the whole
+    state and the trace equal the stored tiles' (ode_split 2) bit for bit over
+    300 steps, and the oracle's in the same arithmetic."""
+    say(f"small-network sweep {prec} L{L}")
+    n = 1000
+    rp, col, w = I.gen_symmetric(29, n, 0.1, 0.02, 0.06)
+    gcal = I.gen_uniform(29, 2, n, 0.5, 1.1)
+    iapp = I.gen_uniform(29, 3, n, -1.0, 3.0)
+    net = make_net(n, prec, rp, col, w, gcal, iapp)
+    if prec == "fp64":   # the FP64 default stays tiled
+        net.set_exec(gather=I.GATHER_SWEEP, tile_cells=64, lds_kb=8, ode_split=0, mode=I.EXEC_GRAPH)
+    e = net.exec
+    assert e["gather"] == I.GATHER_SWEEP and e["tile_cells"] == 64 and e["ode_split"] == 0
+    assert e["lanes_per_cell"] == 8 or prec == "fp64"
+    net.set_exec(lanes_per_cell=L)
+    a = net.advance(300, trace_stride=3)
+    sa = net.state()
+    net.close()
+    # one sender chunk holding the whole network, as the sweep's
+    net = make_net(n, prec, rp, col, w, gcal, iapp, gather=I.GATHER_TILED, lanes_per_cell=L, tile_cells=1024,
+                   lds_kb=8, ode_split=2, block=1024, mode=I.EXEC_GRAPH)
+    b = net.advance(300, trace_stride=3)
+    sb = net.state()
+    net.close()
+    assert np.array_equal(a, b) and np.array_equal(sa, sb)
+    ref, _ = oracle_ref(prec, n, 60, rp, col, w, gcal, iapp, stride=3)
+    tol = 1e-9 if prec == "fp64" else 2e-4
+    assert np.abs(a[:len(ref)] - ref).max() < tol
+
+
 def test_swept_tiles_fp64_degree_budget():
     """FP32 compensated with a degree budget of 80: the swept tiles sum the
     high-degree cells' terms in FP64 as the stored tiles do, bit for bit,
