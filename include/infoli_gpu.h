@@ -258,7 +258,10 @@ int infoli_gap_currents_noalloc(infoli_t *ctx, double *i_c);
 #define INFOLI_EXEC_AUTO        0  /* autotuned / heuristic choice           */
 #define INFOLI_EXEC_STEP        1  /* one fused kernel per step              */
 #define INFOLI_EXEC_GRAPH       2  /* per-step kernels replayed as hipGraph  */
-#define INFOLI_EXEC_PERSISTENT  3  /* multi-step kernel with grid barrier    */
+#define INFOLI_EXEC_PERSISTENT  3  /* steps_per_launch steps in one kernel:
+                                      whole networks per workgroup with no
+                                      barrier between workgroups (GATHER_NETS),
+                                      or the fused gathers with a grid barrier */
 
 /* How a step reads the neighbours' V_dend. */
 #define INFOLI_GATHER_DIRECT    0  /* random loads from the V_dend array        */
