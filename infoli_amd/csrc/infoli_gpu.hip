@@ -141,6 +141,7 @@ struct infoli_ctx {
     int64_t nets_nr = 0;      // ranges = workgroups
     int nets_block = 0;       // threads per workgroup
     int nets_cpt = 1;         // cells per thread
+    bool nets_split = false;  // one thread per compartment (k_persist_nets3)
     size_t nets_lds = 0;      // dynamic LDS per workgroup
     int64_t nets_in_lds = 0;  // ranges whose lists are in LDS
 
@@ -352,6 +353,10 @@ const void *const k_persist_tab[3][3][5] = {
     INFOLI_G_TABLE(k_persist, float, false),
     INFOLI_G_TABLE(k_persist, float, true),
 };
+/* [precision]: one thread per compartment */
+const void *const k_persist_nets3_tab[3] = {(const void *)k_persist_nets3<double, false>,
+                                            (const void *)k_persist_nets3<float, false>,
+                                            (const void *)k_persist_nets3<float, true>};
 /* [precision][cells per thread - 1] (FP64: one cell per thread) */
 const void *const k_persist_nets_tab[3][2] = {
     {(const void *)k_persist_nets<double, false, 1>, (const void *)k_persist_nets<double, false, 1>},
@@ -554,7 +559,7 @@ int launch_persist(infoli_ctx *c, const void *args, int nsteps, int grid)
 {
     if (c->exec.gather == GATHER_NETS) {
         // whole networks per workgroup: no grid barrier, nothing to reset
-        const void *fn = k_persist_nets_tab[c->prec][c->nets_cpt - 1];
+        const void *fn = c->nets_split ? k_persist_nets3_tab[c->prec] : k_persist_nets_tab[c->prec][c->nets_cpt - 1];
         const NetsRange *rng = c->d_rng;
         const unsigned char *lists = c->d_nets_lists;
         int ns = nsteps;
@@ -1050,6 +1055,7 @@ std::vector<int64_t> independent_cuts(const infoli_ctx *c, int64_t cap)
 struct NetsPlan {
     std::vector<int32_t> rng;   // range bounds, nr + 1
     int block = 0, cpt = 1;
+    bool split = false;         // one thread per compartment (k_persist_nets3)
     double max_work = 0, total_work = 0;
 };
 
@@ -1114,6 +1120,10 @@ bool plan_nets(const infoli_ctx *c, NetsPlan &pl)
     }
     pl.cpt = maxc > 1024 ? 2 : 1;
     pl.block = (int)round_up((maxc + pl.cpt - 1) / pl.cpt, 64);
+    // ranges of at most 320 cells: one thread per compartment (three groups
+    // of the range's cells rounded up to 64, at most 960 threads)
+    pl.split = maxc <= 320;
+    if (pl.split) pl.block = 3 * (int)round_up(maxc, 64);
     return true;
 }
 
@@ -1128,8 +1138,10 @@ template <typename T>
 int upload_nets_t(infoli_ctx *c, const NetsPlan &pl)
 {
     const int64_t n = c->n, nr = (int64_t)pl.rng.size() - 1;
-    const int stride = pl.block * pl.cpt;
-    const size_t vbytes = 2 * (size_t)stride * sizeof(T);
+    // lists stride: the cells a workgroup can hold; LDS before the lists:
+    // V_dend in two halves, or all three voltages when split
+    const int stride = pl.split ? pl.block / 3 : pl.block * pl.cpt;
+    const size_t vbytes = (pl.split ? 6 : 2) * (size_t)stride * sizeof(T);
     std::vector<int32_t> deg(n);
     for (int64_t i = 0; i < n; i++) deg[i] = (int32_t)(c->row_ptr[i + 1] - c->row_ptr[i]);
     std::vector<int32_t> swid((n + 63) / 64, 0);   // slice widths of the one-lane layout
@@ -1191,6 +1203,7 @@ int upload_nets_t(infoli_ctx *c, const NetsPlan &pl)
     c->nets_nr = nr;
     c->nets_block = pl.block;
     c->nets_cpt = pl.cpt;
+    c->nets_split = pl.split;
     c->nets_lds = lds_max;
     c->nets_in_lds = in_lds;
     c->nets_valid = true;

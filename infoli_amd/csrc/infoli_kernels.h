@@ -1101,6 +1101,166 @@ __global__ __launch_bounds__(1024) void k_persist_nets(StepArgs<T> a, const Nets
     }
 }
 
+/* The same with one thread per compartment (ranges of at most 320 cells,
+ * three groups of ncp = blockDim / 3 threads): threads [0, ncp) hold the
+ * dendrites and sum the rows, [ncp, 2 ncp) the somata, [2 ncp, 3 ncp) the
+ * axons, each with its compartment's state in registers and the three
+ * voltages of every cell in LDS (two halves each, by step parity).  A
+ * compartment reads only start-of-step voltages, so the three groups run
+ * side by side and the step is the arithmetic of `ode` (and of k_tile_ode3),
+ * bit for bit; each thread's dependent chain is a third of a whole cell's,
+ * which the few wavefronts of a small range cannot otherwise hide.  Lists:
+ * as k_persist_nets, with stride ncp. */
+template <typename T, bool KAHAN>
+__global__ __launch_bounds__(1024) void k_persist_nets3(StepArgs<T> a, const NetsRange *__restrict__ rg,
+                                                        const unsigned char *__restrict__ lists, int nsteps)
+{
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
+    const int ncp = blockDim.x / 3, tid = threadIdx.x;
+    const int part = tid / ncp, loc = tid - part * ncp;   // part: 0 dendrite, 1 soma, 2 axon
+    const NetsRange R = rg[blockIdx.x];
+    const int64_t r0 = R.r0;
+    const int nc = R.r1 - R.r0;
+    T *vdl = reinterpret_cast<T *>(dyn_lds);   // [2][ncp]
+    T *vsl = vdl + 2 * ncp;                    // [2][ncp]
+    T *val = vsl + 2 * ncp;                    // [2][ncp]
+    const uint16_t *ids = reinterpret_cast<const uint16_t *>(val + 2 * ncp);
+    const T *wl = reinterpret_cast<const T *>(dyn_lds + 6 * (size_t)ncp * sizeof(T) +
+                                              (((size_t)R.wmax * ncp * 2 + 15) & ~(size_t)15));
+    if (R.lds) {
+        const size_t bytes = (((size_t)R.wmax * ncp * 2 + 15) & ~(size_t)15) +
+                             (R.wconst ? 0 : (size_t)R.wmax * ncp * sizeof(T));
+        const uint4 *src = reinterpret_cast<const uint4 *>(lists + R.off);
+        uint4 *dst = reinterpret_cast<uint4 *>(val + 2 * ncp);
+        for (size_t i = tid; i < bytes / 16; i += blockDim.x) dst[i] = src[i];
+    }
+    const T wc = (T)R.wc;
+    const int64_t t0 = *a.d_step;
+    const int64_t np = a.npad;
+    const T *st = a.st;
+    const int64_t cell = r0 + loc;
+    const bool live = loc < nc;
+    typename CellOf<T>::type c;
+    T ib = T(0);
+    bool mk = false;
+    int dg = 0, W = 0;
+    int64_t base = 0;
+    if (live) {
+        const int p = (int)(t0 & 1);
+        if (part == 0) {
+            c.r = st[CALCIUM_R * np + cell];
+            c.s = st[POTASSIUM_S * np + cell];
+            c.q = st[HCURRENT_Q * np + cell];
+            c.ca = st[CA2PLUS * np + cell];
+            c.icah = st[I_CAH * np + cell];
+            vdl[loc] = a.vd[p][cell];
+            ib = a.iapp_base ? a.iapp_base[cell] : T(0);
+            mk = !a.mask || a.mask[cell];
+            dg = a.deg[cell];
+            const int64_t s_ = cell >> 6;
+            const int64_t b0 = a.slice_ptr[s_];
+            W = (int)((a.slice_ptr[s_ + 1] - b0) >> 6);
+            base = b0 + (cell & 63);
+        } else if (part == 1) {
+            c.gcal = st[G_CAL * np + cell];
+            c.h = st[SODIUM_H * np + cell];
+            c.n = st[POTASSIUM_N * np + cell];
+            c.p = st[POTASSIUM_P * np + cell];
+            c.xs = st[POTASSIUM_X_S * np + cell];
+            c.k = st[CALCIUM_K * np + cell];
+            c.l = st[CALCIUM_L * np + cell];
+            c.m = 0;
+            vsl[loc] = a.vs[p][cell];
+        } else {
+            c.ha = st[SODIUM_H_A * np + cell];
+            c.xa = st[POTASSIUM_X_A * np + cell];
+            c.ma = 0;
+            val[loc] = a.va[p][cell];
+        }
+    }
+    __syncthreads();
+    for (int k = 0; k < nsteps; k++) {
+        const int64_t t = t0 + k;
+        const int cu = k & 1, nx = cu ^ 1;
+        const T *__restrict__ vc = vdl + cu * ncp;
+        if (live) {
+            if (part == 0) {
+                Acc<T, KAHAN> acc;
+                const T vi = vc[loc];
+                if (dg > 0) {
+                    if (R.lds) {
+                        for (int kk0 = 0; kk0 < W; kk0 += kUnroll) {
+                            int jv[kUnroll];
+                            T wv[kUnroll], vv[kUnroll];
+#pragma unroll
+                            for (int u = 0; u < kUnroll; u++) {
+                                if (kk0 + u < W) {
+                                    jv[u] = ids[(kk0 + u) * ncp + loc];
+                                    wv[u] = R.wconst ? wc : wl[(kk0 + u) * ncp + loc];
+                                }
+                            }
+#pragma unroll
+                            for (int u = 0; u < kUnroll; u++)
+                                if (kk0 + u < W) vv[u] = vc[jv[u]];
+#pragma unroll
+                            for (int u = 0; u < kUnroll; u++) {
+                                if (kk0 + u < W) {
+                                    const T term = gj_term(vi, vv[u], wv[u]);
+                                    acc.add((kk0 + u) < dg ? term : T(0));
+                                }
+                            }
+                        }
+                    } else {
+                        row_terms_idx<T, KAHAN, 1, false>(acc, vi, dg, 0, W, base, a.w, a.col,
+                                                          [&](int j) { return vc[j - r0]; });
+                    }
+                }
+                T iapp;
+                if (a.tab && t >= a.tab_t0 && t < a.tab_t0 + a.tab_n)
+                    iapp = a.tab[(t - a.tab_t0) * a.n + cell];
+                else
+                    iapp = ((t >= a.t0) && (t < a.t1) && mk) ? ib + a.amp : ib;
+                vdl[nx * ncp + loc] = dend<false>(a.m, c, vi, vsl[cu * ncp + loc], acc.value(), iapp);
+            } else if (part == 1) {
+                vsl[nx * ncp + loc] = soma<false>(a.m, c, vc[loc], vsl[cu * ncp + loc], val[cu * ncp + loc]);
+            } else {
+                const T va = axon<false>(a.m, c, vsl[cu * ncp + loc], val[cu * ncp + loc]);
+                val[nx * ncp + loc] = va;
+                if (a.trace && (t % a.stride) == 0) a.trace[((t / a.stride) % a.trace_cap) * a.n + cell] = va;
+            }
+        }
+        __syncthreads();
+    }
+    // the state after the last step, as store_cell leaves it (no trace)
+    if (live && nsteps > 0) {
+        const int64_t tl = t0 + nsteps - 1;
+        const int nx = (int)((tl & 1) ^ 1), lx = nsteps & 1;   // LDS half holding the last step's voltages
+        T *sw = a.st;
+        if (part == 0) {
+            a.vd[nx][cell] = vdl[lx * ncp + loc];
+            sw[CALCIUM_R * np + cell] = c.r;
+            sw[POTASSIUM_S * np + cell] = c.s;
+            sw[HCURRENT_Q * np + cell] = c.q;
+            sw[CA2PLUS * np + cell] = c.ca;
+            sw[I_CAH * np + cell] = c.icah;
+        } else if (part == 1) {
+            a.vs[nx][cell] = vsl[lx * ncp + loc];
+            sw[SODIUM_M * np + cell] = c.m;
+            sw[SODIUM_H * np + cell] = c.h;
+            sw[POTASSIUM_N * np + cell] = c.n;
+            sw[POTASSIUM_P * np + cell] = c.p;
+            sw[POTASSIUM_X_S * np + cell] = c.xs;
+            sw[CALCIUM_K * np + cell] = c.k;
+            sw[CALCIUM_L * np + cell] = c.l;
+        } else {
+            a.va[nx][cell] = val[lx * ncp + loc];
+            sw[SODIUM_M_A * np + cell] = c.ma;
+            sw[SODIUM_H_A * np + cell] = c.ha;
+            sw[POTASSIUM_X_A * np + cell] = c.xa;
+        }
+    }
+}
+
 /* I_c of every cell from V_dend at step t (no state change); direct gathers
  * over the uniform 256-cell blocks of the sliced-ELL layout. */
 template <typename T, int L, bool KAHAN, bool HYB = false>

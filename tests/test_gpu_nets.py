@@ -74,6 +74,25 @@ def test_nets_bitwise_equal_direct_one_lane(prec):
     assert np.array_equal(st, st1)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "fp32k"])
+def test_nets_compartment_threads_bitwise_equal_direct_one_lane(prec):
+    """Networks of at most 300 cells (ranges of at most 320 cells run one
+    thread per compartment): 300 uneven networks with a pulse mask and an
+    Iapp table, 350 steps; state and trace equal the direct gather's at one
+    lane per cell, bit for bit."""
+    batch = uneven_batch(seed=21, n_nets=300, lo=1, hi=300, p=0.1)
+    n = batch[0]
+    mask = (I.gen_uniform(6, 4, n, 0.0, 1.0) < 0.4).astype(np.uint8)
+    tab = I.gen_uniform(7, 6, 30 * n, -0.5, 0.5).reshape(30, n)
+    kw = dict(pulse=(3.0, 40, 250, mask), table=(200, tab))
+    say(f"nets (one thread per compartment) vs direct L1 {prec}: {n} cells")
+    e, tr, st = run(prec, batch, 350, 5, NETS, **kw)
+    assert e["gather"] == I.GATHER_NETS
+    _, tr1, st1 = run(prec, batch, 350, 5, DIRECT1, **kw)
+    assert np.array_equal(tr, tr1)
+    assert np.array_equal(st, st1)
+
+
 @pytest.mark.parametrize("prec,steps,tol", [("fp64", 300, 1e-9), ("fp32k", 60, 2e-4)])
 def test_nets_vs_oracle(prec, steps, tol):
     """A batch of uneven networks against the oracle in the same arithmetic:
