@@ -545,6 +545,92 @@ def test_edge_shapes(prec):
     net.close()
 
 
+def _irregular_lists(n=1000, seed=23):
+    """A random network whose lists hold what the reference's inputs can give:
+    a self-junction on every 7th cell (a nonzero diagonal entry of the dense
+    matrix, src/single_threaded/infoli.c:158-191; d = 0 makes its term 0)
+    and a repeated pair on every 5th cell (a line given twice in a triplet
+    file, src/regression_test/stable_version.c:226-252, both summed), each
+    kept at its place in the list."""
+    rp, col, w, gcal, iapp = _random_case(n, p=0.03, seed=seed)
+    cols, ws = [], []
+    for i in range(n):
+        c, ww = list(col[rp[i]:rp[i + 1]]), list(w[rp[i]:rp[i + 1]])
+        if i % 7 == 0:
+            c.insert(len(c) // 2, i)
+            ww.insert(len(ww) // 2, 0.05)
+        if i % 5 == 0 and c:
+            c.append(c[0])
+            ww.append(ww[0])
+        cols.append(c)
+        ws.append(ww)
+    rp2 = np.zeros(n + 1, np.int64)
+    rp2[1:] = np.cumsum([len(c) for c in cols])
+    return (rp2, np.concatenate(cols).astype(np.int32), np.concatenate(ws).astype(np.float64), gcal, iapp)
+
+
+IRREGULAR_EXECS = [
+    ("direct L1", dict(mode=I.EXEC_GRAPH, gather=I.GATHER_DIRECT, lanes_per_cell=1)),
+    ("direct L4", dict(mode=I.EXEC_GRAPH, gather=I.GATHER_DIRECT, lanes_per_cell=4)),
+    ("staged", dict(mode=I.EXEC_GRAPH, gather=I.GATHER_STAGED, lds_kb=76, lanes_per_cell=2)),
+    ("tiled", dict(mode=I.EXEC_GRAPH, gather=I.GATHER_TILED, lds_kb=1, tile_cells=128, block=256,
+                   lanes_per_cell=1, ode_split=2)),
+    ("swept", dict(mode=I.EXEC_GRAPH, gather=I.GATHER_SWEEP, lds_kb=1, tile_cells=256, block=1024,
+                   lanes_per_cell=1, ode_split=0)),
+    ("nets", dict(mode=I.EXEC_PERSISTENT, gather=I.GATHER_NETS)),
+]
+
+
+@pytest.mark.parametrize("name,exec_kw", IRREGULAR_EXECS, ids=[e[0] for e in IRREGULAR_EXECS])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32k", 2e-4)])
+def test_self_junctions_and_repeated_pairs_vs_oracle(name, exec_kw, prec, tol):
+    """Self-junctions and repeated pairs in the neighbour lists, through every
+    gather (several 1 KiB chunks for the tiled and swept layouts): the trace
+    and the final V_dend match the oracle run on the same lists, 300 steps."""
+    n, steps = 1000, 300
+    rp, col, w, gcal, iapp = _irregular_lists(n)
+    say(f"irregular lists {name} {prec}: {len(col)} entries")
+    net = make_net(n, prec, rp, col, w, gcal, iapp)
+    net.set_exec(steps_per_launch=50, **exec_kw)
+    assert net.exec["gather"] == exec_kw["gather"]
+    out = net.advance(steps, trace_stride=10)
+    vd = net.get_state("V_dend")
+    net.close()
+    ref, st = oracle_ref(prec, n, steps, rp, col, w, gcal, iapp, stride=10)
+    assert np.abs(out - ref).max() < tol
+    assert np.abs(vd - st[O.V_DEND]).max() < tol
+
+
+def test_malformed_lists_refused():
+    """Lists the library cannot run are refused with INFOLI_ERR_ARG before
+    anything changes: a sender outside [0, n), a decreasing row_ptr, a
+    row_ptr not starting at 0, a non-finite conductance; the network then
+    still runs the lists it had."""
+    n = 50
+    rp, col, w = I.gen_symmetric(5, n, 0.2, 0.02, 0.06)
+    net = make_net(n, "fp32k", rp, col, w)
+    ref = net.advance(20, trace_stride=5)
+    bad = []
+    c = col.copy(); c[3] = n; bad.append((rp, c, w))
+    c = col.copy(); c[0] = -1; bad.append((rp, c, w))
+    r = rp.copy(); r[10] = r[11] + 1; bad.append((r, col, w))
+    r = rp.copy(); r[0] = 1; bad.append((r, col, w))
+    ww = w.copy(); ww[7] = np.nan; bad.append((rp, col, ww))
+    ww = w.copy(); ww[8] = np.inf; bad.append((rp, col, ww))
+    for k, (r, c, ww) in enumerate(bad):
+        with pytest.raises(I.InfoliError) as ei:
+            net.set_connectivity(r, c, ww)
+        assert ei.value.code == I.ERR_ARG, k
+    net.close()
+    net = make_net(n, "fp32k", rp, col, w)
+    for r, c, ww in bad:
+        with pytest.raises(I.InfoliError):
+            net.set_connectivity(r, c, ww)
+    out = net.advance(20, trace_stride=5)
+    net.close()
+    assert np.array_equal(out, ref)
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "fp32k"])
 def test_synchrony_bitwise(prec):
     say(f"synchrony {prec}")
