@@ -631,6 +631,96 @@ def test_malformed_lists_refused():
     assert np.array_equal(out, ref)
 
 
+def _coo_to_csr(n, rows, cols, ws):
+    order = np.lexsort((cols, rows))
+    rp = np.zeros(n + 1, np.int64)
+    rp[1:] = np.cumsum(np.bincount(rows, minlength=n))
+    return rp, cols[order].astype(np.int32), ws[order].astype(np.float64)
+
+
+def _random_shape(seed):
+    """A network of a seeded random shape, as the default strategy meets
+    them: kind 0 a symmetric G(n, p) of mean degree up to 200 (constant or
+    spread conductances) of 1-20,000 cells, 1 a sparse base of 2,000-8,000
+    cells with up to five hubs joined to 30-90% of them (degrees past the FP64
+    degree budget), 2 a batch of up to 40 independent networks of 1-1,500
+    cells with its partition, 3 the reference's directed generator
+    (src/openmp/infoli.c:337-375) on 1-6,000 cells; the sizes log-uniform,
+    the kind seed % 4.  Returns (kind, n, rp, col, w, offsets)."""
+    u = I.gen_uniform(seed, 11, 5, 0.0, 1.0)
+    kind = seed % 4
+    n = max(1, int(round(np.exp(u[1] * np.log(20000 if kind == 0 else 6000)))))
+    if kind == 0:
+        md = u[2] * 200
+        p = min(1.0, md / max(1, n - 1))
+        rp, col, w = I.gen_symmetric(seed, n, p, 0.04) if u[3] < 0.5 else I.gen_symmetric(seed, n, p, 0.02, 0.06)
+        return kind, n, rp, col, w, None
+    if kind == 1:
+        n = int(2000 + u[1] * 6000)
+        rp, col, w = I.gen_symmetric(seed, n, min(1.0, 5.0 / (n - 1)), 0.02, 0.06)
+        rows = [np.repeat(np.arange(n), np.diff(rp))]
+        cols, ws = [col.astype(np.int64)], [w]
+        for h in range(1 + int(u[2] * 5)):
+            hub = int(I.gen_uniform(seed, 30 + h, 1, 0, n - 1)[0])
+            frac = 0.3 + 0.6 * I.gen_uniform(seed, 40 + h, 1, 0.0, 1.0)[0]
+            m = np.nonzero(I.gen_uniform(seed, 50 + h, n, 0.0, 1.0) < frac)[0]
+            m = m[m != hub]
+            # a hub's conductances sum to about 60: far stronger coupling
+            # (summed conductance past ~200) makes the forward-Euler step
+            # itself diverge, in the oracle as on the GPU
+            wh = I.gen_uniform(seed, 60 + h, len(m), 0.02, 0.06) * min(1.0, 1500.0 / max(1, len(m)))
+            rows += [np.full(len(m), hub), m]
+            cols += [m, np.full(len(m), hub)]
+            ws += [wh, wh]
+        rp, col, w = _coo_to_csr(n, np.concatenate(rows), np.concatenate(cols), np.concatenate(ws))
+        return kind, n, rp, col, w, None
+    if kind == 2:
+        k = 2 + int(u[2] * 38)
+        sizes = np.maximum(1, np.round(np.exp(I.gen_uniform(seed, 70, k, 0.0, np.log(1500))))).astype(np.int64)
+        offs = np.zeros(k + 1, np.int64)
+        offs[1:] = np.cumsum(sizes)
+        rps, cols, ws, base = [np.zeros(1, np.int64)], [], [], 0
+        ps = I.gen_uniform(seed, 71, k, 0.0, 0.3)
+        for j in range(k):
+            r, c, ww = I.gen_symmetric(seed * 100 + j, int(sizes[j]), float(ps[j]), 0.02, 0.06)
+            rps.append(r[1:] + base)
+            cols.append(c + offs[j])
+            ws.append(ww)
+            base += int(r[-1])
+        n = int(offs[-1])
+        return kind, n, np.concatenate(rps), np.concatenate(cols).astype(np.int32), np.concatenate(ws), offs
+    rp, col, w = I.gen_directed(seed, n, u[2] * 0.2, 0.04)
+    return kind, n, rp, col, w, None
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_default_strategy_random_shapes_vs_oracle(seed):
+    """Sixteen seeded networks of random shape and size, four of each kind,
+    each with per-cell g_CaL and Iapp, advanced by whatever strategy the
+    library picks by default (FP32 compensated and FP64 for two of each
+    kind), 200 steps in two advances: the trace and the final V_dend match
+    the oracle."""
+    kind, n, rp, col, w, offs = _random_shape(seed)
+    prec = "fp32k" if (seed // 4) % 2 == 0 else "fp64"
+    tol = 5e-4 if prec == "fp32k" else 1e-9
+    gcal = I.gen_uniform(seed, 2, n, 0.5, 1.1)
+    iapp = I.gen_uniform(seed, 3, n, -1.0, 3.0)
+    net = I.Network(n, precision=prec)
+    if offs is not None:
+        net.set_partition(offs)
+    net.set_connectivity(rp, col, w)
+    net.set_gcal(gcal)
+    net.set_iapp(iapp)
+    say(f"random shape {seed}: kind {kind}, {n} cells, {len(col)} entries, {prec}, exec {net.exec}")
+    out = np.concatenate([net.advance(k, trace_stride=10) for k in (120, 80)])
+    vd = net.get_state("V_dend")
+    net.close()
+    ref, st = oracle_ref(prec, n, 200, rp, col, w, gcal, iapp, stride=10)
+    assert np.isfinite(ref).all()
+    assert np.abs(out - ref).max() < tol
+    assert np.abs(vd - st[O.V_DEND]).max() < tol
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "fp32k"])
 def test_synchrony_bitwise(prec):
     say(f"synchrony {prec}")
