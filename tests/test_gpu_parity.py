@@ -1,7 +1,7 @@
 """GPU parity: the HIP path (through the C ABI) against the CPU oracle, the
 reference's golden traces and size-independent properties.
 
-Tolerances (max |dV_axon| in mV, the figure BASELINE states):
+Tolerances (max |dV_axon| in mV, the figure of the precision target):
   FP64 GPU vs FP64 reference/oracle: 1e-9 (libm exp differs from the
       device exp by an ulp; the golden single-cell trace has a spike);
   FP32 / FP32+compensated GPU vs FP64: as the FP32 oracle itself
