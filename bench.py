@@ -242,8 +242,8 @@ def main():
                     help="skip the batched_512 leg (all networks in one context vs per-network FP64)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              f"(multi-GPU sharding is described in DESIGN.md, not implemented)", file=sys.stderr)
+        print(f"bench.py: the library runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(sharding a network over GPUs by cell ranges is designed, not implemented)", file=sys.stderr)
         sys.exit(2)
 
     import infoli_amd as I
