@@ -1410,15 +1410,22 @@ __device__ __forceinline__ void tile_row_terms(Acc<T, KAHAN> &acc, double &accd,
             }
         }
     };
+    // Per-entry conductances (not WC): a padding slot holds conductance 0,
+    // so its term is an exact zero (a finite product with 0), and adding it
+    // leaves the sum and its compensation as the degree check's +0 would,
+    // bit for bit; no slot needs the check.  The tile's constant conductance
+    // (WC) and the FP64 terms of wide rows (HYB) keep it.
+    constexpr bool PADZ = !WC && !HYB;
     int kk0 = kstart;
     if constexpr (sizeof(T) == 4) {   // FP64: the extra copies of the loop cost more registers than they save
         // slots below nvm (the fewest valid slots of any lane of the wave)
         // need no check; then whole batches inside the width
-        const int nv = nvm < W ? nvm : W;
+        const int nv = PADZ ? W : (nvm < W ? nvm : W);
         for (; kk0 + kUnroll <= nv; kk0 += kUnroll) batch(std::false_type{}, std::false_type{}, kk0);
-        for (; kk0 + kUnroll <= W; kk0 += kUnroll) batch(std::false_type{}, std::true_type{}, kk0);
+        if constexpr (!PADZ)
+            for (; kk0 + kUnroll <= W; kk0 += kUnroll) batch(std::false_type{}, std::true_type{}, kk0);
     }
-    for (; kk0 < W; kk0 += kUnroll) batch(std::true_type{}, std::true_type{}, kk0);
+    for (; kk0 < W; kk0 += kUnroll) batch(std::true_type{}, std::bool_constant<!PADZ>{}, kk0);
 }
 
 /* FP32 tile rows of constant conductance wc, the ids of the next batch
@@ -1857,7 +1864,7 @@ __device__ __forceinline__ void sweep_tile_rows(const StepArgs<T> &a, const Tile
             const int d = a0.d, W = a0.W;
             Acc<T, KAHAN> acc;
             int nvm = 0;
-            if (sizeof(T) == 4 && W > K) {
+            if (sizeof(T) == 4 && W > K && (wct || HYB)) {
                 int m = d > l ? (d - l + L - 1) / L : 0;
 #pragma unroll
                 for (int off = 32; off > 0; off >>= 1) m = min(m, __shfl_xor(m, off));
@@ -1871,16 +1878,25 @@ __device__ __forceinline__ void sweep_tile_rows(const StepArgs<T> &a, const Tile
 #pragma unroll
                 for (int u = 0; u < K; u++)
                     if (u < W) vv[u] = vch[b0.jv[u]];
+                if (!wct && !(HYB && wide)) {
+                    // per-entry conductances: padding slots add exact zeros
+                    // (tile_row_terms), no degree check
 #pragma unroll
-                for (int u = 0; u < K; u++) {
-                    if (u < W) {
-                        const bool ok = u * L + l < d;
-                        if (HYB && wide) {
-                            const double term = gj_term((double)vi, (double)vv[u], (double)(wct ? wc : b0.wv[u]));
-                            accd += ok ? term : 0.0;
-                        } else {
-                            const T term = gj_term(vi, vv[u], wct ? wc : b0.wv[u]);
-                            acc.add(ok ? term : T(0));
+                    for (int u = 0; u < K; u++)
+                        if (u < W) acc.add(gj_term(vi, vv[u], b0.wv[u]));
+                } else {
+#pragma unroll
+                    for (int u = 0; u < K; u++) {
+                        if (u < W) {
+                            const bool ok = u * L + l < d;
+                            if (HYB && wide) {
+                                const double term =
+                                    gj_term((double)vi, (double)vv[u], (double)(wct ? wc : b0.wv[u]));
+                                accd += ok ? term : 0.0;
+                            } else {
+                                const T term = gj_term(vi, vv[u], wct ? wc : b0.wv[u]);
+                                acc.add(ok ? term : T(0));
+                            }
                         }
                     }
                 }
