@@ -1784,8 +1784,9 @@ __global__ __launch_bounds__(NT, (sizeof(T) == 4 && !HYB) ? (AWC ? 4 : 8) : 1) v
  * wavefront runs its slices as a pipeline, loading the receiver's V_dend
  * with the first K = 4 id / conductance slots DB = 1 slice ahead, and its
  * receiver, degree and slice bounds one slice before that; longer rows load
- * the rest in place (tile_row_terms from slot K), and only they need the
- * wavefront's fewest valid slots.  Measured at 1M cells of degree 50, with
+ * the rest in place (tile_row_terms from slot K), and only they, in tiles
+ * of constant conductance or rows of FP64 terms, need the wavefront's
+ * fewest valid slots.  Measured at 1M cells of degree 50, with
  * the compartments in a kernel after the sweep: 275 us per step; 6 slots
  * ahead 277, 8 slots 285 (305 with the minimum on every slice), two slices
  * ahead 297, four slices' loads at once before summing them 336 (8 slots).
@@ -1863,13 +1864,11 @@ __device__ __forceinline__ void sweep_tile_rows(const StepArgs<T> &a, const Tile
         if (sl < nsl) {
             const int d = a0.d, W = a0.W;
             Acc<T, KAHAN> acc;
+            // the fewest valid slots of any lane: the tile lists receivers by
+            // in-chunk degree, descending, so the slice's last row (lane 63)
+            // has the smallest degree, as in tile_rows
             int nvm = 0;
-            if (sizeof(T) == 4 && W > K && (wct || HYB)) {
-                int m = d > l ? (d - l + L - 1) / L : 0;
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1) m = min(m, __shfl_xor(m, off));
-                nvm = __builtin_amdgcn_readfirstlane(m);
-            }
+            if (sizeof(T) == 4 && W > K && (wct || HYB)) nvm = __builtin_amdgcn_readlane(d, 63) / L;
             if (d > 0) {
                 const T vi = b0.vi;
                 const bool wide = HYB && a.deg[cellb + a0.rloc] >= a.fp64_deg;
