@@ -301,9 +301,11 @@ typedef struct infoli_exec {
     int     ode_split;       /* tiled compartment update: 0 one thread per
                                 cell; 1 one wavefront per compartment; 2 somata,
                                 axons and dendrite gates beside the tiles, the
-                                dendrite voltages after (same results; not
-                                with the sweep, where 0 updates each block's
-                                cells in the sweeping kernel itself)         */
+                                dendrite voltages after (same results).  With
+                                the sweep: 0 each block's cells in the
+                                sweeping kernel itself, 1 a kernel after it,
+                                2 a kernel running beside it (a second graph
+                                branch) and the dendrite voltages after both */
     int     fast_math;       /* tiled / sweep, FP32: hardware exp2 and reciprocal in
                                 the compartment update (0 = libm expf and IEEE
                                 division, the reference's arithmetic)        */

@@ -178,6 +178,9 @@ struct infoli_ctx {
     GraphKey gkey;
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // swept tiles with ode_split 2: the compartment kernel's branch of a step
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipEvent_t ev_copy[2] = {nullptr, nullptr};   // trace pieces copied into staging halves
     std::vector<hipEvent_t> prof_ev;
     bool profiling = false;
@@ -420,6 +423,13 @@ const void *const k_tile_dend_tab[2][3] = {
     {(const void *)k_tile_dend<double, false, false>, (const void *)k_tile_dend<float, false, true>,
      (const void *)k_tile_dend<float, true, true>},
 };
+/* beside the swept tiles: [fast math][precision] */
+const void *const k_tile_beside_tab[2][3] = {
+    {(const void *)k_tile_beside<double, false>, (const void *)k_tile_beside<float, false>,
+     (const void *)k_tile_beside<float, false>},
+    {(const void *)k_tile_beside<double, false>, (const void *)k_tile_beside<float, true>,
+     (const void *)k_tile_beside<float, true>},
+};
 const void *const k_gap_tab[3][5] = {
     INFOLI_L_TABLE(k_gap_currents, double, false),
     INFOLI_L_TABLE(k_gap_currents, float, false),
@@ -503,10 +513,28 @@ int launch_step(infoli_ctx *c, const void *args, int64_t k)
         void *kargs[] = {const_cast<void *>(args), &kk};
         if (c->exec.gather == GATHER_SWEEP) {
             // one workgroup per receiver block; the compartments in the same
-            // kernel (ode_split 0) or after it (1)
+            // kernel (ode_split 0), after it (1), or (2) the somata, axons
+            // and dendrite gates in a kernel beside it and the dendrite
+            // voltages after both
             const void *fs = sweep_kernel(c);
             const size_t lds = sweep_lds_bytes(c, c->tiled_R, c->tl_C);
             allow_dyn_lds(fs, lds);
+            if (c->exec.ode_split == 2) {
+                // fork: the sweep is launched first, so that its workgroups
+                // take the CUs before the smaller ones fill them (the other
+                // order measured 282 against 264 us per step at 1M cells);
+                // join before the dendrite voltages
+                HIPCHK(hipEventRecord(c->ev_fork, c->stream));
+                HIPCHK(hipLaunchKernel(fs, dim3((unsigned)c->tl_nrb), dim3(1024), kargs, lds, c->stream));
+                HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+                HIPCHK(hipLaunchKernel(k_tile_beside_tab[fm][c->prec], dim3((unsigned)(c->npad / 64)), dim3(192),
+                                       kargs, 0, c->side));
+                HIPCHK(hipEventRecord(c->ev_join, c->side));
+                HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+                HIPCHK(hipLaunchKernel(k_tile_dend_tab[fm][c->prec], dim3((unsigned)(c->npad / kBlock)), dim3(kBlock),
+                                       kargs, 0, c->stream));
+                return INFOLI_OK;
+            }
             HIPCHK(hipLaunchKernel(fs, dim3((unsigned)c->tl_nrb), dim3(1024), kargs, lds, c->stream));
             if (c->exec.ode_split == 0) return INFOLI_OK;   // the compartments ran in the sweep
         } else {
@@ -1492,9 +1520,10 @@ void default_exec(infoli_ctx *c)
     // 308 -> 274; 500k cells of degree 50 182 -> 114; 1.5M 599 -> 523; 2M
     // 1008 -> 909.  The direct gather stays ahead at degree 30 (1M cells:
     // 244 -> 265); in FP64 the two are within 3% (575 -> 561 at 1M cells of
-    // degree 50), and the direct gather is kept.  The compartments run in the
-    // sweeping kernel (ode_split 0): 276 -> 274 us at 1M cells, 105 -> 102.5
-    // at 500k, against k_tile_ode3 after it
+    // degree 50), and the direct gather is kept.  The somata, axons and
+    // dendrite gates run in a kernel beside the sweep (ode_split 2): 264 us
+    // at 1M cells, against 271-278 with the compartments in the sweeping
+    // kernel (ode_split 0) and 272 with k_tile_ode3 after it (1)
     if (is_f32(c) && stream_exceeds_cache(c) && mean_deg >= 36.0) {
         const int64_t R = round_up((c->n + c->num_cus - 1) / std::max(c->num_cus, 1), 64);
         const int kb = R <= 8192 ? sweep_max_kb(c, (int)R) : 0;
@@ -1504,7 +1533,7 @@ void default_exec(infoli_ctx *c)
             c->exec.block = 1024;
             c->exec.tile_cells = (int)R;
             c->exec.lds_kb = kb;
-            c->exec.ode_split = 0;
+            c->exec.ode_split = 2;
             return;
         }
     }
@@ -1614,6 +1643,9 @@ extern "C" int infoli_create(infoli_t **out, const infoli_params *params, int64_
         return code;
     };
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_copy[0], hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_copy[1], hipEventDisableTiming) != hipSuccess)
@@ -1668,6 +1700,8 @@ extern "C" void infoli_destroy(infoli_t *c)
     drop_graph(c);
     for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     for (hipEvent_t e : c->ev_copy)
         if (e) (void)hipEventDestroy(e);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -1681,6 +1715,7 @@ extern "C" void infoli_destroy(infoli_t *c)
     if (c->d_nets_lists) (void)hipFree(c->d_nets_lists);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
     if (c->h_status) (void)hipHostFree(c->h_status);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -2050,9 +2085,6 @@ int check_exec(const infoli_ctx *c, infoli_exec &x)
     if (tiled_layout(x) && x.mode == INFOLI_EXEC_PERSISTENT)
         return infoli_fail(INFOLI_ERR_ARG, "the tiled gathers run as two kernels per step; "
                                            "use the step or graph launch mode");
-    if (x.gather == GATHER_SWEEP && x.ode_split == 2)
-        return infoli_fail(INFOLI_ERR_ARG, "the swept tiles fill a CU's LDS, so no somata and axons run beside "
-                                           "them: ode_split 0 or 1");
     if (x.gather == GATHER_VLDS && (size_t)c->n * c->tsize > kVldsMaxBytes)
         return infoli_fail(INFOLI_ERR_ARG, "gather LDS: %lld cells do not fit the LDS budget", (long long)c->n);
     if (x.lds_kb < 1 || x.lds_kb > 156)
@@ -2597,8 +2629,10 @@ extern "C" int infoli_autotune(infoli_t *c, int64_t trial_steps, infoli_exec *ch
                 if (R < 64 || R > 16384 || R % 64 || c->n < (int64_t)16 * R) continue;
                 const int kb = sweep_max_kb(c, R);
                 if (kb < 1) continue;
-                const infoli_exec e{INFOLI_EXEC_GRAPH, L, 1024, spl, GATHER_SWEEP, kb, R, 0, 0};
-                if (memcmp(&e, &h, sizeof e) != 0) cands.push_back(e);
+                for (int split : {0, 2}) {
+                    const infoli_exec e{INFOLI_EXEC_GRAPH, L, 1024, spl, GATHER_SWEEP, kb, R, split, 0};
+                    if (memcmp(&e, &h, sizeof e) != 0) cands.push_back(e);
+                }
             }
         }
     }
@@ -2784,12 +2818,21 @@ int check_tiled(const CsrView &g, size_t tsize, size_t psz, int gather, int L, i
                     return infoli_fail(INFOLI_ERR_STATE, "layout: cell %lld's soma runs %d times, its axon %d (nsa %lld)",
                                        (long long)i, soma[i], axon[i], (long long)nsa);
         }
-    } else if (split == 1) {
+    } else if (split >= 1) {
         for (int64_t rb = 0; rb < nrb; rb++)
             for (int r = 0; r < R; r++)
                 if (!wpart(rb * R + r)) return infoli_fail(INFOLI_ERR_STATE, "layout: block %lld writes past the partials", (long long)rb);
+        if (split == 2) {
+            // k_tile_beside: wave 0 of workgroup b writes the dendrite sums
+            // of cells 64 b .. 64 b + 63
+            for (int64_t b = 0; b < npad / 64; b++)
+                for (int lane = 0; lane < 64; lane++) {
+                    const int64_t cell = b * 64 + lane;
+                    if (cell < n) dw[cell] = dw[npad + cell] = 1;
+                }
+        }
     }
-    const bool reads_parts = gather == GATHER_TILED || split == 1;
+    const bool reads_parts = gather == GATHER_TILED || split >= 1;
     for (int64_t cell = 0; cell < n && reads_parts; cell++) {
         const int64_t rb = cell / R, rloc = cell - rb * R;
         const int64_t t0 = gather == GATHER_TILED ? (any ? tb.rbt0[rb] : 0) : rb;
@@ -2800,7 +2843,7 @@ int check_tiled(const CsrView &g, size_t tsize, size_t psz, int gather, int L, i
                 return infoli_fail(INFOLI_ERR_STATE, "layout: cell %lld reads partial %lld before any kernel of the step writes it",
                                    (long long)cell, (long long)i);
         }
-        if (gather == GATHER_TILED && split == 2 && (!dw[cell] || !dw[npad + cell]))
+        if (split == 2 && (!dw[cell] || !dw[npad + cell]))
             return infoli_fail(INFOLI_ERR_STATE, "layout: cell %lld reads its dendrite sums before they are written", (long long)cell);
     }
     return INFOLI_OK;
@@ -2817,7 +2860,7 @@ extern "C" int infoli_tiled_layout_check(int64_t n, const int64_t *row_ptr, cons
         return infoli_fail(INFOLI_ERR_ARG, "infoli_tiled_layout_check: bad network");
     if (precision < INFOLI_FP64 || precision > INFOLI_FP32_KAHAN || (gather != GATHER_TILED && gather != GATHER_SWEEP) ||
         lidx(lanes_per_cell) < 0 || tile_cells < 64 || tile_cells > 16384 || tile_cells % 64 || lds_kb < 1 ||
-        lds_kb > 156 || ode_split < 0 || ode_split > 2 || (gather == GATHER_SWEEP && ode_split == 2) ||
+        lds_kb > 156 || ode_split < 0 || ode_split > 2 ||
         (block != 256 && block != 512 && block != 1024) || (gather == GATHER_SWEEP && block != 1024) ||
         (ode_split == 2 && nsa < 1))
         return infoli_fail(INFOLI_ERR_ARG, "infoli_tiled_layout_check: bad shape");

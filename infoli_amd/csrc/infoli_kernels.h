@@ -1633,6 +1633,26 @@ __global__ __launch_bounds__(192) void k_tile_ode3(StepArgs<T> a, int64_t k)
     else tile_axon_cell<T, FM>(a, cell, t);
 }
 
+/* Every cell's soma, axon and dendrite up to its voltage (none needs this
+ * step's I_c), in a kernel of its own that runs beside the swept tiles
+ * (k_tile_sweep with ode_split 2).  The sweep fills each CU's LDS, so these
+ * workgroups, which use none, share its CUs through the registers and
+ * wavefront slots it leaves, in a second branch of the step's graph; the
+ * dendrite voltages follow both (k_tile_dend).  192 threads cover 64 cells:
+ * wave 0 the dendrite gates and partial current sum, wave 1 the somata, wave
+ * 2 the axons, from the start-of-step voltages. */
+template <typename T, bool FM>
+__global__ __launch_bounds__(192) void k_tile_beside(StepArgs<T> a, int64_t k)
+{
+    const int64_t t = (a.d_step ? *a.d_step : 0) + k;
+    const int wave = threadIdx.x >> 6;
+    const int64_t cell = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
+    if (cell >= a.n) return;
+    if (wave == 0) tile_dend_pre_cell<T, FM>(a, cell, t);
+    else if (wave == 1) tile_soma_cell<T, FM>(a, cell, t);
+    else tile_axon_cell<T, FM>(a, cell, t);
+}
+
 /* Dendrite voltages (the rest of every compartment ran beside the tiles),
  * one thread per cell: this step's I_c from the tile partials, then the
  * voltage from the partial current sum. */

@@ -213,7 +213,7 @@ def _isolated_blocks_case(n=6000, p=0.01, seed=41, lo=1024, hi=2048):
 @pytest.mark.parametrize("prec,fm", [("fp64", 0), ("fp32", 0), ("fp32k", 0), ("fp32k", 1)])
 @pytest.mark.parametrize("L", [1, 4])
 @pytest.mark.parametrize("R,kb", [(512, 2), (8192, 4)])
-@pytest.mark.parametrize("split", [0, 1])
+@pytest.mark.parametrize("split", [0, 1, 2])
 def test_swept_tiles_equal_tiles(prec, fm, L, R, kb, split):
     """One workgroup per receiver block walking its chunks, each tile's
     (sum, compensation) pair merged in LDS as it comes, gives the stored
@@ -222,7 +222,9 @@ def test_swept_tiles_equal_tiles(prec, fm, L, R, kb, split):
     their sums stay zero; one 8192-cell block holds 64 KiB of sums beside
     its chunk.  ode_split 0 updates the cells inside the sweeping kernel
     (against k_tile_ode after the stored tiles), 1 in k_tile_ode3 after
-    both.  The trace matches the oracle."""
+    both, 2 the somata, axons and dendrite gates in a kernel beside the
+    sweep (against the soma/axon workgroups beside the stored tiles) and the
+    dendrite voltages after.  The trace matches the oracle."""
     say(f"swept tiles {prec} fast_math {fm} L={L} R={R} split {split}")
     n, steps = 6000, 150
     rp, col, w, gcal, iapp = _isolated_blocks_case(n)
@@ -328,8 +330,6 @@ def test_swept_default_large_network_vs_oracle(prec, tol):
 def test_swept_tiles_options_checked():
     say("swept tiles option checks")
     net = make_net(40000, "fp32k")
-    with pytest.raises(I.InfoliError):   # no somata and axons beside the swept tiles
-        net.set_exec(mode=I.EXEC_GRAPH, gather=I.GATHER_SWEEP, tile_cells=512, lds_kb=8, ode_split=2)
     with pytest.raises(I.InfoliError):   # 16384 sums (128 KiB) and a 39 KiB chunk exceed the LDS
         net.set_exec(mode=I.EXEC_GRAPH, gather=I.GATHER_SWEEP, tile_cells=16384, lds_kb=40, ode_split=1)
     with pytest.raises(I.InfoliError):

@@ -35,7 +35,7 @@ def block_diagonal_case(sizes=(70, 300, 1, 130, 900, 64), p=0.2, seed=5):
 
 
 GATHER_SPLITS = [(I.GATHER_TILED, 0), (I.GATHER_TILED, 1), (I.GATHER_TILED, 2),
-                 (I.GATHER_SWEEP, 0), (I.GATHER_SWEEP, 1)]
+                 (I.GATHER_SWEEP, 0), (I.GATHER_SWEEP, 1), (I.GATHER_SWEEP, 2)]
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "fp32k"])
@@ -87,7 +87,7 @@ def test_star_long_rows():
 def test_check_rejects_bad_shapes():
     rp, col, w = isolated_blocks_case(n=600)
     for kw in (dict(lanes_per_cell=3), dict(tile_cells=100), dict(lds_kb=0), dict(ode_split=3),
-               dict(gather=I.GATHER_SWEEP, ode_split=2), dict(gather=I.GATHER_SWEEP, block=512),
+               dict(gather=I.GATHER_SWEEP, block=512),
                dict(ode_split=2, nsa=0), dict(gather=I.GATHER_DIRECT)):
         with pytest.raises(I.InfoliError):
             I.tiled_layout_check(rp, col, w, **kw)
