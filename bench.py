@@ -33,7 +33,7 @@ HBM_PEAK_GBS = 8000.0
 VEXP_PEAK_PER_S = 13.0e12
 GATHER_NAMES = {0: "direct", 1: "lds", 2: "staged", 3: "tiled", 4: "sweep", 5: "nets"}
 TILED_KERNELS = {0: "k_tile_gj+k_tile_ode", 1: "k_tile_gj+k_tile_ode3", 2: "k_tile_gj+k_tile_dend"}
-SWEEP_KERNELS = {0: "k_tile_sweep", 1: "k_tile_sweep+k_tile_ode3"}
+SWEEP_KERNELS = {0: "k_tile_sweep", 1: "k_tile_sweep+k_tile_ode3", 2: "k_tile_sweep+k_tile_beside+k_tile_dend"}
 
 
 def kernel_name(exec_cfg):
