@@ -1077,12 +1077,13 @@ def test_isolated_steps_rand_init_vs_oracle(prec, tol):
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32k"])
 @pytest.mark.parametrize("split_at", [150, 151])
-@pytest.mark.parametrize("swept", [False, True])
+@pytest.mark.parametrize("swept", [None, 0, 2])
 def test_checkpoint_resume_bitwise(prec, split_at, swept, tmp_path):
     """Save after `split_at` steps, continue; a fresh network loading the
     checkpoint continues identically (trace and all 19 variables), whichever
     parity the step count has, under the default gather and under swept
-    tiles with the compartments in the sweeping kernel."""
+    tiles with the compartments in the sweeping kernel (ode_split 0) or in a
+    kernel beside it, a second branch of the step's graph (2)."""
     say(f"checkpoint {prec} at {split_at} swept {swept}")
     n = 500
     rp, col, w, gcal, iapp = _random_case(n, p=0.05, seed=21)
@@ -1092,8 +1093,8 @@ def test_checkpoint_resume_bitwise(prec, split_at, swept, tmp_path):
     def build():
         net = make_net(n, prec, rp, col, w, gcal, iapp)
         net.set_iapp_pulse(3.0, 100, 260, mask)
-        if swept:
-            net.set_exec(mode=I.EXEC_GRAPH, gather=I.GATHER_SWEEP, tile_cells=128, lds_kb=1, ode_split=0,
+        if swept is not None:
+            net.set_exec(mode=I.EXEC_GRAPH, gather=I.GATHER_SWEEP, tile_cells=128, lds_kb=1, ode_split=swept,
                          lanes_per_cell=1, steps_per_launch=40)
         return net
 
