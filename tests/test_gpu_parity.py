@@ -245,6 +245,27 @@ def test_swept_tiles_equal_tiles(prec, fm, L, R, kb, split):
         assert np.abs(outs[1][0] - ref).max() < (1e-10 if prec == "fp64" else 2e-4)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32k"])
+def test_swept_tiles_beside_kernel_step_mode_equals_graph(prec):
+    """The compartments in a kernel beside the sweep (ode_split 2) fork the
+    step onto a second stream and join it before the dendrite voltages: as
+    single launches (step mode) and captured in a graph they give the same
+    trace and state bit for bit, over advances of 37 and 64 steps."""
+    n = 6000
+    rp, col, w, gcal, iapp = _isolated_blocks_case(n)
+    outs = []
+    for mode in (I.EXEC_STEP, I.EXEC_GRAPH):
+        net = make_net(n, prec, rp, col, w, gcal, iapp)
+        net.set_exec(mode=mode, gather=I.GATHER_SWEEP, lds_kb=2, tile_cells=512, lanes_per_cell=1, ode_split=2,
+                     steps_per_launch=20)
+        say(f"beside kernel {prec} mode {mode}")
+        tr = np.concatenate([net.advance(k, trace_stride=3) for k in (37, 64)])
+        outs.append((tr, np.stack([net.get_state(v) for v in I.VARS])))
+        net.close()
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "fp32k"])
 @pytest.mark.parametrize("L", [1, 4, 8])
 def test_small_network_sweep_compartments_beside_rows(prec, L):
