@@ -1519,12 +1519,16 @@ void default_exec(infoli_ctx *c)
     // degree 50 372 -> 275, of degree 100 681 -> 337, of degree 40
     // 308 -> 274; 500k cells of degree 50 182 -> 114; 1.5M 599 -> 523; 2M
     // 1008 -> 909.  The direct gather stays ahead at degree 30 (1M cells:
-    // 244 -> 265); in FP64 the two are within 3% (575 -> 561 at 1M cells of
-    // degree 50), and the direct gather is kept.  The somata, axons and
-    // dendrite gates run in a kernel beside the sweep (ode_split 2): 264 us
-    // at 1M cells, against 271-278 with the compartments in the sweeping
-    // kernel (ode_split 0) and 272 with k_tile_ode3 after it (1)
-    if (is_f32(c) && stream_exceeds_cache(c) && mean_deg >= 36.0) {
+    // 244 -> 265).  The somata, axons and dendrite gates run in a kernel
+    // beside the sweep (ode_split 2): 264 us at 1M cells, against 271-278
+    // with the compartments in the sweeping kernel (ode_split 0) and 272 with
+    // k_tile_ode3 after it (1).  In FP64 (direct -> swept with ode_split 2):
+    // 1M cells of degree 40 483-500 -> 505, of degree 50 573-587 -> 538, of
+    // degree 100 979 -> 662; but 500k cells of degree 50 224 -> 235-251,
+    // where V_dend (4 MB) fits one XCD's 4 MiB L2 and the direct gather's
+    // loads hit it.  So FP64 sweeps from degree 45 when V_dend exceeds 4 MiB
+    const bool sweep64 = !is_f32(c) && mean_deg >= 45.0 && (size_t)c->n * sizeof(double) > ((size_t)4 << 20);
+    if (stream_exceeds_cache(c) && ((is_f32(c) && mean_deg >= 36.0) || sweep64)) {
         const int64_t R = round_up((c->n + c->num_cus - 1) / std::max(c->num_cus, 1), 64);
         const int kb = R <= 8192 ? sweep_max_kb(c, (int)R) : 0;
         if (kb >= 16) {
