@@ -322,9 +322,10 @@ def main():
     steps_per_launch = 1.0
     bytes_per_launch = algorithmic_bytes_per_step(n, nnz, args.precision)
     bytes_note = "8 B per neighbour entry (int32 id + conductance) + 144 B per cell"
-    if wl.get("partition") is not None and exec_cfg["gather"] == 3:
-        # the batched networks' tiles hold 16-bit ids and no conductances
-        # (w = 0.04 everywhere): the step is priced against its own layout
+    if exec_cfg["gather"] in (3, 4) and nnz > 0 and bool(np.all(wl["w"] == wl["w"][0])):
+        # tiles of one conductance everywhere (the batched and dense
+        # networks, w = 0.04) hold 16-bit ids and no conductances: the step
+        # is priced against its own layout, not the 8 B count it beats
         bytes_per_launch = layout_bytes_per_step(n, nnz, args.precision, constant_w=True)
         bytes_note = "tiled layout's own bytes: 2 B per entry (16-bit ids, constant conductance) + 144 B per cell"
     achieved = bytes_per_launch / (mean_launch_ms * 1e-3) / 1e9
