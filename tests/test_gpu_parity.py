@@ -348,6 +348,25 @@ def test_swept_default_large_network_vs_oracle(prec, tol):
     net.close()
 
 
+def test_swept_default_fp64_beyond_one_l2_vs_oracle():
+    """600,000 cells of mean degree 50 in FP64: V_dend (4.8 MB) no longer
+    fits one XCD's 4 MiB L2, so the heuristic sweeps the FP64 tiles too
+    (the somata, axons and dendrite gates in the kernel beside the sweep),
+    and the trace matches the FP64 oracle."""
+    say("swept default 600k fp64")
+    n, steps = 600000, 20
+    rp, col, w = I.gen_symmetric(47, n, 50.0 / (n - 1), 0.02, 0.06)
+    gcal = I.gen_uniform(47, 2, n, 0.5, 1.1)
+    iapp = I.gen_normal(47, 3, n, 1.0, 1.0)
+    net = make_net(n, "fp64", rp, col, w, gcal, iapp)
+    e = net.exec
+    assert e["gather"] == I.GATHER_SWEEP and e["ode_split"] == 2 and e["lanes_per_cell"] == 1
+    out = net.advance(steps, trace_stride=5)
+    net.close()
+    ref, _ = oracle_ref("fp64", n, steps, rp, col, w, gcal, iapp, stride=5)
+    assert np.abs(out - ref).max() < 1e-9
+
+
 def test_swept_tiles_options_checked():
     say("swept tiles option checks")
     net = make_net(40000, "fp32k")
