@@ -298,7 +298,9 @@ def main():
         for prec in ("fp32k", "fp32"):
             print(f"# study {prec} vs fp64, {steps} steps", file=sys.stderr, flush=True)
             a, b = make(prec), make("fp64")
-            res[prec] = analysis.compare_runs(a, b, steps)
+            res[prec] = analysis.compare_runs(
+                a, b, steps, progress=lambda t, dv: print(f"#   step {t}: max |dV| so far {dv:.3e} mV",
+                                                          file=sys.stderr, flush=True))
             res[prec]["exec"] = a.exec
             a.close()
             b.close()

@@ -71,11 +71,12 @@ def spike_agreement(cells_a, steps_a, cells_b, steps_b, window=50):
 
 
 def compare_runs(net_a, net_b, steps, window_steps=2000, threshold=-20.0, chunk=250,
-                 match_window=50):
+                 match_window=50, progress=None):
     """Advance two networks in lockstep for `steps` steps with a V_axon
     record every step: max |V_a - V_b| over the first `window_steps` steps,
     and the spike-time agreement of the steps after them (all steps if the
-    run is not longer than the window)."""
+    run is not longer than the window).  progress(t, maxdv), if given, is
+    called after every 2,000 steps compared."""
     n = net_a.n
     maxdv = 0.0
     prev_a = prev_b = None
@@ -96,6 +97,8 @@ def compare_runs(net_a, net_b, steps, window_steps=2000, threshold=-20.0, chunk=
             ev[key][1].append(s[keep])
         prev_a, prev_b = ta[-1], tb[-1]
         t += k
+        if progress is not None and (t % 2000 == 0 or t == steps):
+            progress(t, maxdv)
     ca, sa = (np.concatenate(x) if x else np.zeros(0, np.int64) for x in ev["a"])
     cb, sb = (np.concatenate(x) if x else np.zeros(0, np.int64) for x in ev["b"])
     out = {"cells": n, "steps": steps, "window_steps": min(window_steps, steps),
