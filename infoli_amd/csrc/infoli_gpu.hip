@@ -163,6 +163,9 @@ struct infoli_ctx {
     int *d_flag = nullptr;               // &d_status->nonfinite
     AdvanceStatus *h_status = nullptr;   // pinned copy read after each advance
     int64_t step = 0;
+    // the device step counter holds `step` and the status is clear, as the
+    // last advance left them: the next advance needs no k_begin
+    bool begin_done = false;
 
     std::vector<int64_t> row_ptr;
     std::vector<int32_t> col;
@@ -2011,6 +2014,7 @@ extern "C" int infoli_load_checkpoint(infoli_t *c, const char *path)
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->step = h.step;   // first: the voltages go to the buffers of this step's parity
+    c->begin_done = false;
     for (int j = 0; j < INFOLI_NVARS; j++) RCHK(infoli_set_state(c, j, v.data() + (size_t)j * c->n));
     return INFOLI_OK;
 }
@@ -2386,8 +2390,15 @@ int advance_impl(infoli_ctx *c, int64_t n_steps, int64_t stride, double *trace)
     for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
     c->prof_ev.clear();
 
-    hipLaunchKernelGGL(k_begin, dim3(1), dim3(1), 0, c->stream, c->d_step, c->step, c->d_status);
-    HIPCHK(hipGetLastError());
+    // k_begin sets the device step counter and clears the status, unless the
+    // last advance left both so (it ran on the counter and ended clean): one
+    // launch less in front of the steps of back-to-back advances
+    const bool need_begin = !c->begin_done;
+    c->begin_done = false;
+    if (need_begin) {
+        hipLaunchKernelGGL(k_begin, dim3(1), dim3(1), 0, c->stream, c->d_step, c->step, c->d_status);
+        HIPCHK(hipGetLastError());
+    }
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     int64_t t = c->step, left = n_steps, rec = 0;
     int64_t pending = 0;   // records of the previous piece still in staging buffer (piece - 1) & 1
@@ -2431,6 +2442,10 @@ int advance_impl(infoli_ctx *c, int64_t n_steps, int64_t stride, double *trace)
     }
     if (gb.error) return infoli_fail(INFOLI_ERR_BARRIER, "persistent kernel grid barrier timed out");
     if (flag) return infoli_fail(INFOLI_ERR_NONFINITE, "a voltage is NaN or Inf after step %lld", (long long)c->step);
+    // steps launched one by one (launch mode STEP, or graph mode while
+    // profiling) take their index as an argument and leave the counter behind
+    const bool on_counter = c->exec.mode != INFOLI_EXEC_STEP && !(c->profiling && c->exec.mode == INFOLI_EXEC_GRAPH);
+    c->begin_done = on_counter && gb.count == 0;
     return INFOLI_OK;
 }
 
@@ -2683,6 +2698,7 @@ extern "C" int infoli_autotune(infoli_t *c, int64_t trial_steps, infoli_exec *ch
     c->exec = best_e;
     drop_graph(c);
     c->step = step0;
+    c->begin_done = false;
     for (int v = 0; v < kNV; v++)
         RCHK(h2d(c, var_ptr(c, v), saved.data() + plane * v, plane));
     c->profiling = prof;

@@ -525,6 +525,59 @@ def test_trace_split_calls_equal_single_call():
     a.close(); b.close()
 
 
+def test_step_counter_across_modes_and_calls(tmp_path):
+    """Back-to-back advances start from the device step counter the last
+    one left; steps launched one by one (launch mode STEP, or graph mode
+    while profiling), a checkpoint load and the autotuner leave it behind,
+    and the next advance sets it again.  A pulse window that starts and ends
+    mid-call makes a wrong step index show in the trace: every mix of calls
+    equals one 120-step graph advance bit for bit."""
+    say("step counter across launch modes and calls")
+    n = 300
+    rp, col, w, gcal, iapp = _random_case(n, seed=9)
+    mask = (np.arange(n) % 3 == 0).astype(np.uint8)
+
+    def net():
+        x = make_net(n, "fp32k", rp, col, w, gcal, iapp)
+        x.set_iapp_pulse(6.0, 37, 83, mask)
+        x.set_exec(mode=I.EXEC_GRAPH, steps_per_launch=10)
+        return x
+
+    a = net()
+    ref = a.advance(120, trace_stride=1)
+    a.close()
+    b = net()
+    parts = [b.advance(20, trace_stride=1), b.advance(15, trace_stride=1)]
+    b.set_exec(mode=I.EXEC_STEP)
+    parts.append(b.advance(9, trace_stride=1))
+    b.set_exec(mode=I.EXEC_GRAPH, steps_per_launch=10)
+    parts.append(b.advance(16, trace_stride=1))
+    b.set_profiling(True)
+    parts.append(b.advance(10, trace_stride=1))
+    b.set_profiling(False)
+    parts.append(b.advance(10, trace_stride=1))
+    assert b.step_count == 80
+    assert np.array_equal(np.concatenate(parts), ref[:80])
+    b.close()
+    c = net()
+    c.advance(30, trace_stride=1)
+    ck = str(tmp_path / "at30.ck")
+    c.save_checkpoint(ck)
+    c.advance(20, trace_stride=1)          # on to step 50, then back to 30
+    c.load_checkpoint(ck)
+    tail = c.advance(40, trace_stride=1)
+    assert c.step_count == 70 and np.array_equal(tail, ref[30:70])
+    c.close()
+    d = net()
+    d.advance(30, trace_stride=1)
+    e0 = d.exec
+    d.autotune(trial_steps=20)              # trial steps run, then the state goes back to step 30
+    d.set_exec(**e0)
+    tail = d.advance(40, trace_stride=1)
+    assert d.step_count == 70 and np.array_equal(tail, ref[30:70])
+    d.close()
+
+
 def test_state_roundtrip_and_iapp_table():
     say("state roundtrip")
     n = 257
