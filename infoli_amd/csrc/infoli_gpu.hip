@@ -2254,8 +2254,8 @@ int run_steps(infoli_ctx *c, StepArgs<T> a, int64_t t, int64_t count)
         return INFOLI_OK;
     }
     // the device step counter holds t here: k_begin set it at the start of
-    // the advance, each replay adds its steps, and so do the bumps after
-    // steps launched one by one
+    // the advance (or the last advance left it there), each replay adds its
+    // steps, and so do the bumps after steps launched one by one
     const int G = c->exec.steps_per_launch;
     a.d_step = c->d_step;
     int64_t done = 0;
